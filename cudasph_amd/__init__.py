@@ -1,0 +1,265 @@
+"""MI355X-native SPH engine: Python bindings over libcudasph.so.
+
+The compute path is the HIP library built from ``cudasph_amd/csrc`` (see
+``cudasph_amd.build``).  This module only marshals arrays through the C ABI of
+``include/cudasph.h``; it has no numerical fallback and raises if the library
+is missing or the GPU call fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libcudasph.so"
+
+FORM_PLAIN, FORM_MIXED, FORM_FP64 = 0, 1, 2
+FORMS = {"plain": FORM_PLAIN, "mixed": FORM_MIXED, "fp64": FORM_FP64}
+
+# pass ids (include/cudasph.h)
+(TREE, SOLVE, OFFSETS, FIND, DENSITY, EOS, HYDRO, REACTION, GRAVITY, ADD_GRAVITY, TIMESTEP,
+ PREDICT, CORRECT, STATS, PH_SORT, SPH_ACC, INIT, STEP) = range(18)
+
+
+class Params(C.Structure):
+    """sph_params (include/cudasph.h); same layout as the oracle's orc_params."""
+
+    _fields_ = [
+        ("n_bodies", C.c_int), ("n_dev", C.c_int), ("rel_err", C.c_float),
+        ("desired_ngb", C.c_float), ("max_iter", C.c_int), ("fix_phantom", C.c_int),
+        ("av_alpha", C.c_float), ("av_beta", C.c_float), ("cn_vel", C.c_float),
+        ("cn_eth", C.c_float), ("grav_mode", C.c_int), ("grav_eps", C.c_float),
+        ("eos_density_unit", C.c_double), ("eos_energy_unit", C.c_double),
+        ("eos_pressure_unit", C.c_double), ("eos_amu", C.c_double),
+        ("eos_boltzmann", C.c_double), ("eos_rad_const", C.c_double),
+        ("eos_gamma", C.c_double), ("eos_temp_base", C.c_double),
+        ("ph_period", C.c_int), ("ph_bits", C.c_int), ("eos_newton", C.c_int),
+        ("n_threads", C.c_int),
+    ]
+
+    def copy(self) -> "Params":
+        p = Params()
+        C.memmove(C.byref(p), C.byref(self), C.sizeof(Params))
+        return p
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libcudasph.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m cudasph_amd.build` first")
+    L = C.CDLL(str(LIB_PATH))
+    vp, cp, i, l, d = C.c_void_p, C.c_char_p, C.c_int, C.c_long, C.c_double
+    L.sph_params_default.argtypes = [C.POINTER(Params), i]
+    L.sph_params_preset.argtypes = [C.POINTER(Params), i, i]
+    L.sph_params_preset.restype = i
+    L.sph_ic_generate.argtypes = [i, i, C.c_uint64, vp, vp, vp]
+    L.sph_ic_generate.restype = i
+    L.sph_last_error.restype = cp
+    L.sph_engine_create.argtypes = [i, C.POINTER(Params), i]
+    L.sph_engine_create.restype = vp
+    L.sph_engine_destroy.argtypes = [vp]
+    L.sph_engine_set.argtypes = [vp, cp, vp, l]
+    L.sph_engine_set.restype = l
+    L.sph_engine_get.argtypes = [vp, cp, vp, l]
+    L.sph_engine_get.restype = l
+    L.sph_engine_get_scalar.argtypes = [vp, cp]
+    L.sph_engine_get_scalar.restype = d
+    L.sph_engine_set_scalar.argtypes = [vp, cp, d]
+    L.sph_engine_set_scalar.restype = i
+    for fn in ("sph_engine_pass", "sph_engine_run"):
+        getattr(L, fn).argtypes = [vp, i]
+        getattr(L, fn).restype = i
+    L.sph_engine_sync.argtypes = [vp]
+    L.sph_engine_sync.restype = i
+    L.sph_engine_stats.argtypes = [vp, C.POINTER(C.c_double)]
+    L.sph_engine_stats.restype = i
+    L.sph_engine_timers.argtypes = [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_long), i]
+    L.sph_engine_timers.restype = i
+    L.sph_engine_reset_timers.argtypes = [vp]
+    L.sph_engine_read_state.argtypes = [vp, cp]
+    L.sph_engine_read_state.restype = i
+    L.sph_engine_write_state.argtypes = [vp, cp]
+    L.sph_engine_write_state.restype = i
+    L.sph_state_file_count.argtypes = [cp]
+    L.sph_state_file_count.restype = i
+    L.sph_abi_set_params.argtypes = [C.POINTER(Params), i]
+    _lib = L
+    return L
+
+
+def default_params(n_bodies: int) -> Params:
+    p = Params()
+    lib().sph_params_default(C.byref(p), int(n_bodies))
+    return p
+
+
+def preset_params(config: int, n_bodies: int = 0) -> Params:
+    p = Params()
+    if lib().sph_params_preset(C.byref(p), int(config), int(n_bodies)) != 0:
+        raise ValueError(f"unknown config {config}")
+    return p
+
+
+def generate_ic(config: int, n: int, seed: int = 12345):
+    """Seeded initial conditions in FP64: pos4 {x,y,z,range}, vel4 {v,m}, hydro4 {rho,P,u,mu}."""
+    pos = np.zeros((n, 4), np.float64)
+    vel = np.zeros((n, 4), np.float64)
+    hyd = np.zeros((n, 4), np.float64)
+    if lib().sph_ic_generate(int(config), int(n), int(seed), pos.ctypes.data, vel.ctypes.data,
+                             hyd.ctypes.data) != 0:
+        raise ValueError(f"ic generation failed for config {config}")
+    return pos, vel, hyd
+
+
+def _err(what: str):
+    return RuntimeError(f"{what}: {lib().sph_last_error().decode()}")
+
+
+STATS_KEYS = ("T", "U", "W", "E", "J", "Fx", "Fy", "Fz", "Tx", "Ty", "Tz")
+
+
+class Engine:
+    """A device-resident simulation (one HIP stream on one GPU)."""
+
+    def __init__(self, form, params: Params, device: int = 0):
+        self.form = FORMS[form] if isinstance(form, str) else int(form)
+        self.params = params.copy()
+        self.n = params.n_bodies
+        self.nd = params.n_dev
+        self.real = np.float64 if self.form == FORM_FP64 else np.float32
+        h = lib().sph_engine_create(self.form, C.byref(self.params), int(device))
+        if not h:
+            raise _err("sph_engine_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sph_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # array layout by name
+    def _spec(self, name: str):
+        r, nd = self.real, self.nd
+        cap = int(100 * nd)
+        table = {
+            "pos": (r, (nd, 4)), "vel": (r, (nd, 4)), "hydro": (r, (nd, 4)), "dots": (r, (nd, 4)),
+            "bdots": (r, (nd, 4)), "hydro2": (r, (nd, 4)), "grav": (r, (nd, 4)),
+            "tree": (r, (nd + 2, 4)), "hash": (np.uint32, (nd, 2)), "n_ngb": (np.int32, (nd,)),
+            "ngb_count": (np.int32, (nd,)), "ngb_offset": (np.int32, (nd + 1,)),
+            "ngb_list": (np.int32, (cap,)), "ngb_acc": (r, (cap,)),
+            "pos64": (np.float64, (nd, 4)), "vel64": (np.float64, (nd, 4)), "u64": (np.float64, (nd,)),
+            "pos_lo": (np.float32, (nd, 4)),
+        }
+        return table[name]
+
+    def get(self, name: str, count: int | None = None) -> np.ndarray:
+        dt, shape = self._spec(name)
+        out = np.zeros(shape, dt)
+        nb = out.nbytes if count is None else int(count) * out.itemsize * (out.shape[1] if out.ndim > 1 else 1)
+        got = lib().sph_engine_get(self._h, name.encode(), out.ctypes.data, nb)
+        if got < 0:
+            raise _err(f"get {name}")
+        return out if count is None else out[:count]
+
+    def set(self, name: str, arr) -> None:
+        dt, _ = self._spec(name)
+        a = np.ascontiguousarray(arr, dtype=dt)
+        if lib().sph_engine_set(self._h, name.encode(), a.ctypes.data, a.nbytes) < 0:
+            raise _err(f"set {name}")
+
+    def load_ic(self, pos, vel, hyd) -> None:
+        """Load FP64 initial conditions (n records); padding records keep their defaults."""
+        n = self.n
+        for name, src in (("pos", pos), ("vel", vel), ("hydro", hyd)):
+            cur = self.get(name)
+            cur[:n] = np.asarray(src)[:n]
+            self.set(name, cur)
+        if self.form == FORM_MIXED:
+            p64 = np.zeros((self.nd, 4))
+            p64[:n, :3] = pos[:n, :3]
+            v64 = np.zeros((self.nd, 4))
+            v64[:n] = vel[:n]
+            u64 = np.zeros(self.nd)
+            u64[:n] = hyd[:n, 2]
+            self.set("pos64", p64)
+            self.set("vel64", v64)
+            self.set("u64", u64)
+
+    def run_pass(self, pid: int) -> None:
+        if lib().sph_engine_pass(self._h, int(pid)) != 0:
+            raise _err(f"pass {pid}")
+
+    def init(self) -> None:
+        self.run_pass(INIT)
+
+    def step(self, n: int = 1) -> None:
+        if lib().sph_engine_run(self._h, int(n)) != 0:
+            raise _err("run")
+
+    def sync(self) -> None:
+        if lib().sph_engine_sync(self._h) != 0:
+            raise _err("sync")
+
+    def scalar(self, name: str) -> float:
+        return lib().sph_engine_get_scalar(self._h, name.encode())
+
+    def set_scalar(self, name: str, v: float) -> None:
+        if lib().sph_engine_set_scalar(self._h, name.encode(), float(v)) != 0:
+            raise _err(f"set_scalar {name}")
+
+    def stats(self) -> dict:
+        out = (C.c_double * 11)()
+        if lib().sph_engine_stats(self._h, out) != 0:
+            raise _err("stats")
+        return dict(zip(STATS_KEYS, list(out)))
+
+    def timers(self) -> dict:
+        k = 64
+        names = (C.c_char_p * k)()
+        ms = (C.c_double * k)()
+        calls = (C.c_long * k)()
+        m = lib().sph_engine_timers(self._h, names, ms, calls, k)
+        if m < 0:
+            raise _err("timers")
+        return {names[i].decode(): (ms[i], calls[i]) for i in range(min(m, k))}
+
+    def reset_timers(self) -> None:
+        lib().sph_engine_reset_timers(self._h)
+
+    def write_state(self, path: str) -> None:
+        if lib().sph_engine_write_state(self._h, str(path).encode()) != 0:
+            raise _err("write_state")
+
+    def read_state(self, path: str) -> None:
+        if lib().sph_engine_read_state(self._h, str(path).encode()) != 0:
+            raise _err("read_state")
+
+
+def make_engine(form, config: int, n: int | None = None, seed: int = 12345, params: Params | None = None,
+                device: int = 0) -> Engine:
+    """Engine loaded with the seeded initial conditions of a benchmark config."""
+    p = params if params is not None else preset_params(config, n or 0)
+    if (FORMS[form] if isinstance(form, str) else form) != FORM_PLAIN:
+        p = p.copy()
+        p.eos_newton = 1
+    e = Engine(form, p, device)
+    pos, vel, hyd = generate_ic(config, p.n_bodies, seed)
+    e.load_ic(pos, vel, hyd)
+    return e

@@ -1,0 +1,233 @@
+// Engine C API (group 2 of include/cudasph.h): exception-safe wrappers over
+// the engine, the reference state-file format, and the last-error string.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cudasph.h"
+#include "engine.h"
+
+using namespace cudasph;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+auto guard(F&& f, decltype(f()) fail) -> decltype(f()) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+  } catch (...) {
+    g_last_error = "unknown error";
+  }
+  return fail;
+}
+
+EngineBase* E(void* e) {
+  if (!e) throw std::invalid_argument("null engine");
+  return static_cast<EngineBase*>(e);
+}
+
+// reference state file: header (read_state.cpp:5-23) + 23 doubles per body
+// (read_state.cpp:34-67)
+struct StateHeader {
+  int n_bodies, nnopt;
+  double hmin, hmax, sep0, t_end, dt_out;
+  int snapshot_number, iteration_number;
+  double global_time;
+  int nav;
+  double alpha, beta, eta2;
+  int ngr, nrelax;
+  double trelax, dt, omega2;
+};
+
+bool read_header(FILE* f, StateHeader& h) {
+  bool ok = true;
+  auto rd = [&](void* p, size_t n) { ok = ok && std::fread(p, n, 1, f) == 1; };
+  rd(&h.n_bodies, 4); rd(&h.nnopt, 4);
+  rd(&h.hmin, 8); rd(&h.hmax, 8); rd(&h.sep0, 8); rd(&h.t_end, 8); rd(&h.dt_out, 8);
+  rd(&h.snapshot_number, 4); rd(&h.iteration_number, 4);
+  rd(&h.global_time, 8);
+  rd(&h.nav, 4);
+  rd(&h.alpha, 8); rd(&h.beta, 8); rd(&h.eta2, 8);
+  rd(&h.ngr, 4); rd(&h.nrelax, 4);
+  rd(&h.trelax, 8); rd(&h.dt, 8); rd(&h.omega2, 8);
+  return ok;
+}
+void write_header(FILE* f, const StateHeader& h) {
+  auto wr = [&](const void* p, size_t n) { std::fwrite(p, n, 1, f); };
+  wr(&h.n_bodies, 4); wr(&h.nnopt, 4);
+  wr(&h.hmin, 8); wr(&h.hmax, 8); wr(&h.sep0, 8); wr(&h.t_end, 8); wr(&h.dt_out, 8);
+  wr(&h.snapshot_number, 4); wr(&h.iteration_number, 4);
+  wr(&h.global_time, 8);
+  wr(&h.nav, 4);
+  wr(&h.alpha, 8); wr(&h.beta, 8); wr(&h.eta2, 8);
+  wr(&h.ngr, 4); wr(&h.nrelax, 4);
+  wr(&h.trelax, 8); wr(&h.dt, 8); wr(&h.omega2, 8);
+}
+
+constexpr double kAmuCgs = 1.66054e-24;   // read_state.cpp:61, write_state.cpp:53
+
+template <class T>
+void read_state_into(EngineBase* e, FILE* f, const StateHeader& h, int nd) {
+  const int n = h.n_bodies;
+  std::vector<V4<T>> pos(nd), vel(nd), hyd(nd), bd(nd), gr(nd), dd(nd);
+  e->get("pos", pos.data(), (long)(nd * sizeof(V4<T>)));
+  e->get("vel", vel.data(), (long)(nd * sizeof(V4<T>)));
+  e->get("hydro", hyd.data(), (long)(nd * sizeof(V4<T>)));
+  e->get("dots", dd.data(), (long)(nd * sizeof(V4<T>)));
+  std::vector<double> rec(23);
+  for (int i = 0; i < n; ++i) {
+    if (std::fread(rec.data(), 8, 23, f) != 23) throw std::runtime_error("state file truncated");
+    pos[i] = {T(rec[0]), T(rec[1]), T(rec[2]), T(2 * rec[4])};
+    vel[i] = {T(rec[6]), T(rec[7]), T(rec[8]), T(rec[3])};
+    hyd[i] = {T(rec[5]), T(0), T(rec[12]), T(rec[18] / kAmuCgs)};
+    bd[i] = {T(rec[9]), T(rec[10]), T(rec[11]), T(rec[13])};
+    gr[i] = {T(rec[14]), T(rec[15]), T(rec[16]), T(rec[17])};
+    dd[i].x = T(rec[22]);
+  }
+  e->set("pos", pos.data(), (long)(nd * sizeof(V4<T>)));
+  e->set("vel", vel.data(), (long)(nd * sizeof(V4<T>)));
+  e->set("hydro", hyd.data(), (long)(nd * sizeof(V4<T>)));
+  e->set("bdots", bd.data(), (long)(nd * sizeof(V4<T>)));
+  e->set("grav", gr.data(), (long)(nd * sizeof(V4<T>)));
+  e->set("dots", dd.data(), (long)(nd * sizeof(V4<T>)));
+  if (e->get_scalar("form") == SPH_FORM_MIXED) {
+    std::vector<D4> p64(nd), v64(nd);
+    std::vector<double> u64(nd);
+    std::fseek(f, -(long)n * 23 * 8, SEEK_CUR);
+    for (int i = 0; i < n; ++i) {
+      if (std::fread(rec.data(), 8, 23, f) != 23) throw std::runtime_error("state file truncated");
+      p64[i] = {rec[0], rec[1], rec[2], 0.0};
+      v64[i] = {rec[6], rec[7], rec[8], rec[3]};
+      u64[i] = rec[12];
+    }
+    e->set("pos64", p64.data(), (long)(nd * sizeof(D4)));
+    e->set("vel64", v64.data(), (long)(nd * sizeof(D4)));
+    e->set("u64", u64.data(), (long)(nd * sizeof(double)));
+  }
+}
+
+template <class T>
+void write_state_from(EngineBase* e, FILE* f, int n, int nd) {
+  std::vector<V4<T>> pos(nd), vel(nd), hyd(nd), bd(nd), gr(nd), dd(nd);
+  e->get("pos", pos.data(), (long)(nd * sizeof(V4<T>)));
+  e->get("vel", vel.data(), (long)(nd * sizeof(V4<T>)));
+  e->get("hydro", hyd.data(), (long)(nd * sizeof(V4<T>)));
+  e->get("bdots", bd.data(), (long)(nd * sizeof(V4<T>)));
+  e->get("grav", gr.data(), (long)(nd * sizeof(V4<T>)));
+  e->get("dots", dd.data(), (long)(nd * sizeof(V4<T>)));
+  for (int i = 0; i < n; ++i) {
+    const double rec[23] = {(double)pos[i].x, (double)pos[i].y, (double)pos[i].z, (double)vel[i].w,
+                            (double)pos[i].w * 0.5, (double)hyd[i].x, (double)vel[i].x, (double)vel[i].y,
+                            (double)vel[i].z, (double)bd[i].x, (double)bd[i].y, (double)bd[i].z,
+                            (double)hyd[i].z, (double)bd[i].w, (double)gr[i].x, (double)gr[i].y,
+                            (double)gr[i].z, (double)gr[i].w, (double)hyd[i].w * kAmuCgs, -1.0, -1.0, -1.0,
+                            (double)dd[i].x};
+    std::fwrite(rec, 8, 23, f);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sph_last_error(void) { return g_last_error.c_str(); }
+
+void* sph_engine_create(int form, const sph_params* p, int device_id) {
+  return guard([&]() -> void* { return make_engine(form, *p, device_id); }, (void*)nullptr);
+}
+
+void sph_engine_destroy(void* e) {
+  guard([&] { delete static_cast<EngineBase*>(e); return 0; }, 0);
+}
+
+long sph_engine_set(void* e, const char* name, const void* host, long n_bytes) {
+  return guard([&] { return E(e)->set(name, host, n_bytes); }, -2L);
+}
+long sph_engine_get(void* e, const char* name, void* host, long max_bytes) {
+  return guard([&] { return E(e)->get(name, host, max_bytes); }, -2L);
+}
+double sph_engine_get_scalar(void* e, const char* name) {
+  return guard([&] { return E(e)->get_scalar(name); }, (double)NAN);
+}
+int sph_engine_set_scalar(void* e, const char* name, double v) {
+  return guard([&] { E(e)->set_scalar(name, v); return 0; }, -1);
+}
+int sph_engine_pass(void* e, int pass) {
+  return guard([&] { E(e)->pass(pass); return 0; }, -1);
+}
+int sph_engine_run(void* e, int steps) {
+  return guard([&] { E(e)->run(steps); return 0; }, -1);
+}
+int sph_engine_sync(void* e) {
+  return guard([&] { E(e)->sync(); return 0; }, -1);
+}
+int sph_engine_stats(void* e, double* out11) {
+  return guard([&] { E(e)->stats(out11); return 0; }, -1);
+}
+int sph_engine_timers(void* e, const char** names, double* ms, long* calls, int max_n) {
+  return guard([&] { return E(e)->timers(names, ms, calls, max_n); }, -1);
+}
+void sph_engine_reset_timers(void* e) {
+  guard([&] { E(e)->reset_timers(); return 0; }, 0);
+}
+
+int sph_state_file_count(const char* path) {
+  return guard([&] {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) throw std::runtime_error(std::string("cannot open file ") + path);
+    StateHeader h;
+    const bool ok = read_header(f, h);
+    std::fclose(f);
+    if (!ok) throw std::runtime_error("state header truncated");
+    return h.n_bodies;
+  }, -1);
+}
+
+int sph_engine_read_state(void* e, const char* path) {
+  return guard([&] {
+    EngineBase* en = E(e);
+    FILE* f = std::fopen(path, "rb");
+    if (!f) throw std::runtime_error(std::string("cannot open file ") + path);
+    StateHeader h;
+    if (!read_header(f, h)) { std::fclose(f); throw std::runtime_error("state header truncated"); }
+    const sph_params& p = en->params();
+    if (h.n_bodies != p.n_bodies) { std::fclose(f); throw std::runtime_error("state file N differs from the engine's"); }
+    if (en->get_scalar("real_bytes") == 8) read_state_into<double>(en, f, h, p.n_dev);
+    else read_state_into<float>(en, f, h, p.n_dev);
+    std::fclose(f);
+    en->set_scalar("dt", h.dt);
+    en->set_scalar("global_time", h.global_time);
+    en->set_scalar("iteration", h.iteration_number);
+    return 0;
+  }, -1);
+}
+
+int sph_engine_write_state(void* e, const char* path) {
+  return guard([&] {
+    EngineBase* en = E(e);
+    const sph_params& p = en->params();
+    FILE* f = std::fopen(path, "wb");
+    if (!f) throw std::runtime_error(std::string("cannot open file ") + path);
+    StateHeader h{};
+    h.n_bodies = p.n_bodies;
+    h.t_end = 0;
+    h.iteration_number = (int)en->get_scalar("iteration");
+    h.global_time = en->get_scalar("global_time");
+    h.alpha = p.av_alpha;
+    h.beta = p.av_beta;
+    h.dt = en->get_scalar("dt");
+    write_header(f, h);
+    if (en->get_scalar("real_bytes") == 8) write_state_from<double>(en, f, p.n_bodies, p.n_dev);
+    else write_state_from<float>(en, f, p.n_bodies, p.n_dev);
+    std::fclose(f);
+    return 0;
+  }, -1);
+}
+
+}  // extern "C"

@@ -1,0 +1,49 @@
+// Shared types and helpers for the HIP kernels and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "types.h"
+
+namespace cudasph {
+
+constexpr int kBlock = 256;        // threads per block for per-particle kernels
+constexpr int kStackDepth = 28;    // kd-walk stack entries per lane: tree height + 1 <= 25 for n < 2^24
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+[[noreturn]] inline void throw_hip(const char* file, int line, const char* expr, hipError_t e) {
+  char buf[512];
+  std::snprintf(buf, sizeof buf, "%s:%d: %s failed: %s", file, line, expr, hipGetErrorString(e));
+  throw HipError(buf);
+}
+
+#define CS_HIP(expr)                                                       \
+  do {                                                                     \
+    hipError_t cs_e_ = (expr);                                             \
+    if (cs_e_ != hipSuccess) cudasph::throw_hip(__FILE__, __LINE__, #expr, cs_e_); \
+  } while (0)
+
+#define CS_CHECK_LAUNCH() CS_HIP(hipGetLastError())
+
+// kd-tree node word, device side (see types.h)
+__device__ __forceinline__ int node_bits(float w) { return __float_as_int(w); }
+__device__ __forceinline__ int node_bits(double w) { return (int)w; }
+
+// n_norm (CUDAsph.h:16-20): round up to a multiple of j, at least j
+__host__ __device__ __forceinline__ int n_norm(int n, int j) {
+  n = ((n - 1) / j) * j + j;
+  if (n == 0) n = j;
+  return n;
+}
+
+}  // namespace cudasph

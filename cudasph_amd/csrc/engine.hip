@@ -1,0 +1,685 @@
+// Device-resident SPH engine: the passes of step.cpp:13-147 over HBM state.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kdtree.h"
+#include "kernels.h"
+#include "sph_math.h"
+
+namespace cudasph {
+
+// ---------------------------------------------------------------------------
+hipEvent_t Timers::take() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  CS_HIP(hipEventCreate(&e));
+  return e;
+}
+void Timers::begin(const char* name, hipStream_t s) {
+  if (!enabled) return;
+  open_ = name;
+  open_ev_ = take();
+  CS_HIP(hipEventRecord(open_ev_, s));
+}
+void Timers::end(hipStream_t s) {
+  if (!enabled || !open_) return;
+  hipEvent_t b = take();
+  CS_HIP(hipEventRecord(b, s));
+  pending_.push_back({open_, open_ev_, b});
+  open_ = nullptr;
+}
+void Timers::resolve() {
+  for (auto& p : pending_) {
+    CS_HIP(hipEventSynchronize(p.b));
+    float ms = 0.f;
+    CS_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    auto& a = acc_[p.name];
+    a.first += ms;
+    a.second += 1;
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  pending_.clear();
+}
+int Timers::list(const char** names, double* ms, long* calls, int max_n) {
+  resolve();
+  int k = 0;
+  for (auto& kv : acc_) {
+    if (k >= max_n) break;
+    // the map key owns the string; hand out a stable pointer
+    names[k] = kv.first.c_str();
+    ms[k] = kv.second.first;
+    calls[k] = kv.second.second;
+    ++k;
+  }
+  return (int)acc_.size();
+}
+void Timers::reset() {
+  resolve();
+  acc_.clear();
+}
+Timers::~Timers() {
+  for (auto& p : pending_) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : pool_) (void)hipEventDestroy(e);
+}
+
+struct TimerScope {
+  Timers& t;
+  hipStream_t s;
+  TimerScope(Timers& t_, const char* name, hipStream_t s_) : t(t_), s(s_) { t.begin(name, s); }
+  ~TimerScope() { t.end(s); }
+};
+
+inline int bits_for(long v) {
+  int b = 0;
+  while ((1L << b) <= v) ++b;
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+template <class T>
+class Engine final : public EngineBase {
+ public:
+  Engine(int form, const sph_params& p, int device);
+  ~Engine() override {
+    if (s_) (void)hipStreamDestroy(s_);
+  }
+  long set(const std::string& name, const void* host, long n_bytes) override;
+  long get(const std::string& name, void* host, long max_bytes) override;
+  double get_scalar(const std::string& name) override;
+  void set_scalar(const std::string& name, double v) override;
+  void pass(int id) override;
+  void run(int steps) override {
+    for (int k = 0; k < steps; ++k) step();
+  }
+  void sync() override { CS_HIP(hipStreamSynchronize(s_)); }
+  void stats(double* out11) override;
+  int timers(const char** names, double* ms, long* calls, int max_n) override {
+    sync();
+    return timers_.list(names, ms, calls, max_n);
+  }
+  void reset_timers() override {
+    sync();
+    timers_.reset();
+  }
+  const sph_params& params() const override { return p_; }
+
+ private:
+  static constexpr bool kF32 = sizeof(T) == 4;
+  bool mixed() const { return form_ == SPH_FORM_MIXED; }
+  WalkParams walk_params() const {
+    return {p_.n_bodies, p_.fix_phantom, p_.rel_err, p_.desired_ngb, p_.max_iter};
+  }
+  EosConst eos_const() const {
+    EosConst c;
+    c.density_unit = p_.eos_density_unit;
+    c.energy_unit = p_.eos_energy_unit;
+    c.pressure_unit = p_.eos_pressure_unit;
+    c.amu = p_.eos_amu;
+    c.boltzmann = p_.eos_boltzmann;
+    c.rad_const = p_.eos_rad_const;
+    c.gamma = p_.eos_gamma;
+    c.temp_base = p_.eos_temp_base;
+    c.newton = p_.eos_newton;
+    return c;
+  }
+  void refresh_if_dirty() {
+    if (mixed() && dirty64_) {
+      launch_refresh_mixed(pos64_, vel64_, u64_, (F4*)pos_.p, pos_lo_, (F4*)vel_.p, (F4*)hydro_.p, n_, s_);
+      dirty64_ = false;
+    }
+  }
+
+  void tree();
+  void solve();
+  void offsets();
+  void find();
+  void density();
+  void eos();
+  void hydro();
+  void reaction();
+  void sph_accelerations();
+  void gravity();
+  void add_gravity();
+  void timestep(double* out);
+  void predict();
+  void correct(const double* dt_new, const double* dt_old);
+  void stats_pass();
+  void sort_bodies();
+  void init();
+  void step();
+
+  int form_;
+  sph_params p_;
+  int n_, nd_;
+  long cap_;
+  hipStream_t s_ = nullptr;
+  Timers timers_;
+
+  DBuf<V4<T>> pos_, vel_, hydro_, dots_, bdots_, hydro2_, grav_, tree_, gpos_, gvel_, old_vu_, tmp4_;
+  DBuf<KV> hash_, pairs_, pairs_alt_, perm_;
+  DBuf<int> n_ngb_, ngb_count_, ngb_offset_, list_, owner_, tr_start_, tr_end_, flags_, scan_tmp_, sort_tmp_;
+  DBuf<T> acc_;
+  DBuf<double> dt_cur_, dt_next_, partials_, force6_, stats6_;
+  DBuf<uint32_t> key_hi_, key_lo_;
+  // mixed form
+  DBuf<D4> pos64_, vel64_, old64_;
+  DBuf<double> u64_;
+  DBuf<F4> pos_lo_, ghi_, glo_, gvel32_;
+  bool dirty64_ = false;
+
+  HBuf<V4<T>> h_pos_, h_tree_;
+  HBuf<KV> h_hash_;
+  HBuf<long long> h_small_;
+
+  double dt_ = 0, global_time_ = 0;
+  int iteration_ = 0, exact_rsqrt_ = 0;
+  long ngb_tot_ = 0;
+  int overflow_reruns_ = 0;
+};
+
+template <class T>
+Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p) {
+  n_ = p.n_bodies;
+  nd_ = p.n_dev;
+  if (n_ <= 0 || nd_ < n_) throw std::invalid_argument("sph_engine: need 0 < n_bodies <= n_dev");
+  if (nd_ >= (1 << 24) - 2) throw std::invalid_argument("sph_engine: n_dev must be below 2^24 (tree node word)");
+  if (form == SPH_FORM_MIXED && !kF32) throw std::invalid_argument("sph_engine: mixed form is FP32");
+  CS_HIP(hipSetDevice(device));
+  CS_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+  cap_ = 100L * nd_;  // ngb_tot_max (read_state.cpp:29)
+
+  for (auto* b : {&pos_, &vel_, &hydro_, &dots_, &bdots_, &hydro2_, &grav_, &gpos_, &gvel_, &old_vu_, &tmp4_})
+    b->alloc(nd_);
+  tree_.alloc(nd_ + 2);
+  hash_.alloc(nd_);
+  perm_.alloc(nd_);
+  n_ngb_.alloc(nd_);
+  ngb_count_.alloc(nd_);
+  ngb_offset_.alloc(nd_ + 1);
+  list_.alloc(cap_);
+  owner_.alloc(cap_);
+  acc_.alloc(cap_);
+  pairs_.alloc(cap_);
+  pairs_alt_.alloc(cap_);
+  tr_start_.alloc(nd_ + 1);
+  tr_end_.alloc(nd_ + 1);
+  flags_.alloc(4);
+  scan_tmp_.alloc(scan_scratch_ints(std::max<long>(cap_, nd_ + 1)) + 16);
+  sort_tmp_.alloc(radix_scratch_ints(cap_) + 16);
+  dt_cur_.alloc(1);
+  dt_next_.alloc(1);
+  partials_.alloc(reduce_partials(nd_) + 64);
+  force6_.alloc(6);
+  stats6_.alloc(6);
+  key_hi_.alloc(nd_);
+  key_lo_.alloc(nd_);
+  if (mixed()) {
+    pos64_.alloc(nd_);
+    vel64_.alloc(nd_);
+    old64_.alloc(nd_);
+    u64_.alloc(nd_);
+    pos_lo_.alloc(nd_);
+    ghi_.alloc(nd_);
+    glo_.alloc(nd_);
+    gvel32_.alloc(nd_);
+    CS_HIP(hipMemsetAsync(pos_lo_, 0, pos_lo_.bytes(), s_));
+    CS_HIP(hipMemsetAsync(pos64_, 0, pos64_.bytes(), s_));
+    CS_HIP(hipMemsetAsync(vel64_, 0, vel64_.bytes(), s_));
+    CS_HIP(hipMemsetAsync(u64_, 0, u64_.bytes(), s_));
+  }
+  h_pos_.alloc(nd_);
+  h_tree_.alloc(nd_ + 2);
+  h_hash_.alloc(nd_);
+  h_small_.alloc(8);
+
+  for (auto* b : {&pos_, &vel_, &hydro_, &dots_, &bdots_, &hydro2_, &grav_, &gpos_, &gvel_, &old_vu_})
+    CS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), s_));
+  for (auto* b : {&n_ngb_, &ngb_count_, &ngb_offset_, &tr_start_, &tr_end_, &flags_})
+    CS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), s_));
+  CS_HIP(hipMemsetAsync(dt_cur_, 0, 8, s_));
+  CS_HIP(hipMemsetAsync(dt_next_, 0, 8, s_));
+  CS_HIP(hipMemsetAsync(stats6_, 0, 48, s_));
+  CS_HIP(hipMemsetAsync(force6_, 0, 48, s_));
+
+  // identity hash and far padding records (read_state.cpp:70-89)
+  std::vector<KV> hh(nd_);
+  for (int i = 0; i < nd_; ++i) hh[i] = {i < n_ ? (uint32_t)i : (1u << 24), (uint32_t)i};
+  CS_HIP(hipMemcpyAsync(hash_, hh.data(), nd_ * sizeof(KV), hipMemcpyHostToDevice, s_));
+  if (nd_ > n_) {
+    std::vector<V4<T>> far(nd_ - n_, V4<T>{T(1e10), T(1e10), T(1e10), T(0)});
+    CS_HIP(hipMemcpyAsync(pos_.p + n_, far.data(), far.size() * sizeof(V4<T>), hipMemcpyHostToDevice, s_));
+    CS_HIP(hipMemcpyAsync(vel_.p + n_, far.data(), far.size() * sizeof(V4<T>), hipMemcpyHostToDevice, s_));
+  }
+  sync();
+}
+
+template <class T>
+long Engine<T>::set(const std::string& name, const void* host, long nb) {
+  auto put = [&](void* dev, size_t have) {
+    const size_t c = std::min<size_t>(have, (size_t)nb);
+    CS_HIP(hipMemcpyAsync(dev, host, c, hipMemcpyHostToDevice, s_));
+    sync();
+    return (long)c;
+  };
+  if (name == "pos") return put(pos_, pos_.bytes());
+  if (name == "vel") return put(vel_, vel_.bytes());
+  if (name == "hydro") return put(hydro_, hydro_.bytes());
+  if (name == "dots") return put(dots_, dots_.bytes());
+  if (name == "bdots") return put(bdots_, bdots_.bytes());
+  if (name == "hydro2") return put(hydro2_, hydro2_.bytes());
+  if (name == "grav") return put(grav_, grav_.bytes());
+  if (name == "tree") return put(tree_, tree_.bytes());
+  if (name == "hash") return put(hash_, hash_.bytes());
+  if (name == "n_ngb") return put(n_ngb_, n_ngb_.bytes());
+  if (name == "ngb_count") return put(ngb_count_, ngb_count_.bytes());
+  if (name == "ngb_offset") return put(ngb_offset_, ngb_offset_.bytes());
+  if (name == "ngb_list") return put(list_, list_.bytes());
+  if (name == "ngb_acc") return put(acc_, acc_.bytes());
+  if (mixed()) {
+    if (name == "pos64") { dirty64_ = true; return put(pos64_, pos64_.bytes()); }
+    if (name == "vel64") { dirty64_ = true; return put(vel64_, vel64_.bytes()); }
+    if (name == "u64") { dirty64_ = true; return put(u64_, u64_.bytes()); }
+  }
+  return -1;
+}
+
+template <class T>
+long Engine<T>::get(const std::string& name, void* host, long mb) {
+  refresh_if_dirty();
+  auto take = [&](const void* dev, size_t have) {
+    const size_t c = std::min<size_t>(have, (size_t)mb);
+    sync();
+    CS_HIP(hipMemcpy(host, dev, c, hipMemcpyDeviceToHost));
+    return (long)c;
+  };
+  if (name == "pos") return take(pos_, pos_.bytes());
+  if (name == "vel") return take(vel_, vel_.bytes());
+  if (name == "hydro") return take(hydro_, hydro_.bytes());
+  if (name == "dots") return take(dots_, dots_.bytes());
+  if (name == "bdots") return take(bdots_, bdots_.bytes());
+  if (name == "hydro2") return take(hydro2_, hydro2_.bytes());
+  if (name == "grav") return take(grav_, grav_.bytes());
+  if (name == "tree") return take(tree_, tree_.bytes());
+  if (name == "hash") return take(hash_, hash_.bytes());
+  if (name == "n_ngb") return take(n_ngb_, n_ngb_.bytes());
+  if (name == "ngb_count") return take(ngb_count_, ngb_count_.bytes());
+  if (name == "ngb_offset") return take(ngb_offset_, ngb_offset_.bytes());
+  if (name == "ngb_list") return take(list_, list_.bytes());
+  if (name == "ngb_acc") return take(acc_, acc_.bytes());
+  if (mixed()) {
+    if (name == "pos64") return take(pos64_, pos64_.bytes());
+    if (name == "vel64") return take(vel64_, vel64_.bytes());
+    if (name == "u64") return take(u64_, u64_.bytes());
+    if (name == "pos_lo") return take(pos_lo_, pos_lo_.bytes());
+  }
+  return -1;
+}
+
+template <class T>
+double Engine<T>::get_scalar(const std::string& name) {
+  if (name == "dt") {
+    sync();
+    CS_HIP(hipMemcpy(&dt_, dt_cur_, 8, hipMemcpyDeviceToHost));
+    return dt_;
+  }
+  if (name == "global_time") return global_time_;
+  if (name == "iteration") return iteration_;
+  if (name == "ngb_tot") return (double)ngb_tot_;
+  if (name == "overflow_reruns") return overflow_reruns_;
+  if (name == "exact_rsqrt") return exact_rsqrt_;
+  if (name == "timing") return timers_.enabled;
+  if (name == "cap") return (double)cap_;
+  if (name == "real_bytes") return (double)sizeof(T);
+  if (name == "form") return form_;
+  return NAN;
+}
+
+template <class T>
+void Engine<T>::set_scalar(const std::string& name, double v) {
+  if (name == "dt") {
+    sync();
+    dt_ = v;
+    CS_HIP(hipMemcpy(dt_cur_, &dt_, 8, hipMemcpyHostToDevice));
+  } else if (name == "global_time") {
+    global_time_ = v;
+  } else if (name == "iteration") {
+    iteration_ = (int)v;
+  } else if (name == "exact_rsqrt") {
+    exact_rsqrt_ = (int)v;
+  } else if (name == "timing") {
+    sync();
+    timers_.enabled = v != 0;
+  } else {
+    throw std::invalid_argument("unknown scalar " + name);
+  }
+}
+
+template <class T>
+void Engine<T>::stats(double* o) {
+  sync();
+  double s6[6], f6[6];
+  CS_HIP(hipMemcpy(s6, stats6_, 48, hipMemcpyDeviceToHost));
+  CS_HIP(hipMemcpy(f6, force6_, 48, hipMemcpyDeviceToHost));
+  o[0] = s6[0];
+  o[1] = s6[1];
+  o[2] = s6[2];
+  o[3] = s6[0] + s6[1] + s6[2];
+  o[4] = std::sqrt(s6[3] * s6[3] + s6[4] * s6[4] + s6[5] * s6[5]);
+  for (int k = 0; k < 6; ++k) o[5 + k] = f6[k];
+}
+
+// ---------------------------------------------------------------------------
+// passes
+
+// kd-tree build on the host (sph_accelerations.cpp:14-33), reference-exact
+template <class T>
+void Engine<T>::tree() {
+  TimerScope ts(timers_, "tree_build_host", s_);
+  CS_HIP(hipMemcpyAsync(h_pos_.p, pos_, (size_t)nd_ * sizeof(V4<T>), hipMemcpyDeviceToHost, s_));
+  CS_HIP(hipMemcpyAsync(&h_small_[0], dt_cur_, 8, hipMemcpyDeviceToHost, s_));
+  sync();
+  std::memcpy(&dt_, &h_small_[0], 8);
+  build_kd_tree<T>(h_pos_.p, n_, nd_, p_.n_threads, h_tree_.p, h_hash_.p);
+  CS_HIP(hipMemcpyAsync(tree_, h_tree_.p, (size_t)(nd_ + 2) * sizeof(V4<T>), hipMemcpyHostToDevice, s_));
+  CS_HIP(hipMemcpyAsync(hash_, h_hash_.p, (size_t)nd_ * sizeof(KV), hipMemcpyHostToDevice, s_));
+}
+
+template <class T>
+void Engine<T>::solve() {
+  TimerScope ts(timers_, "solve_range", s_);
+  launch_solve_range<T>(pos_, n_ngb_, hash_, tree_, nd_, walk_params(), s_);
+}
+
+template <class T>
+void Engine<T>::offsets() {
+  TimerScope ts(timers_, "offsets_scan", s_);
+  launch_exclusive_scan(n_ngb_, ngb_offset_, nd_, 1, scan_tmp_, s_);
+}
+
+// dev_find_ngb; if a particle finds more ids than its slot holds (the
+// reference's count/list mismatch), rebuild the offsets from the true
+// counts and walk again
+template <class T>
+void Engine<T>::find() {
+  const WalkParams wp = walk_params();
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    {
+      TimerScope ts(timers_, "find_ngb", s_);
+      CS_HIP(hipMemsetAsync(flags_, 0, 16, s_));
+      launch_find_ngb<T>(pos_, tree_, hash_, ngb_offset_, attempt == 0 ? n_ngb_.p : ngb_count_.p, ngb_count_,
+                         list_, flags_, nd_, wp, 0, s_);
+    }
+    int fl = 0, tot = 0;
+    CS_HIP(hipMemcpyAsync(&h_small_[0], flags_, 4, hipMemcpyDeviceToHost, s_));
+    CS_HIP(hipMemcpyAsync(&h_small_[1], ngb_offset_.p + nd_, 4, hipMemcpyDeviceToHost, s_));
+    sync();
+    std::memcpy(&fl, &h_small_[0], 4);
+    std::memcpy(&tot, &h_small_[1], 4);
+    ngb_tot_ = tot;
+    if (ngb_tot_ > cap_) throw std::runtime_error("neighbour list overflow: ngb_tot > 100 n_dev (sph_accelerations.cpp:66-72)");
+    if (!fl) return;
+    ++overflow_reruns_;
+    launch_exclusive_scan(ngb_count_, ngb_offset_, nd_, 1, scan_tmp_, s_);
+    CS_HIP(hipMemcpyAsync(&h_small_[1], ngb_offset_.p + nd_, 4, hipMemcpyDeviceToHost, s_));
+    sync();
+    std::memcpy(&tot, &h_small_[1], 4);
+    if (tot > cap_) throw std::runtime_error("neighbour list overflow after recount");
+  }
+  throw std::runtime_error("find_ngb: slot overflow persisted after recount");
+}
+
+template <class T>
+void Engine<T>::density() {
+  refresh_if_dirty();
+  TimerScope ts(timers_, "density", s_);
+  if constexpr (kF32) {
+    if (mixed())
+      launch_density<float, true>(pos_, vel_, pos_lo_, hydro_, dots_, hash_, ngb_offset_, ngb_count_, list_, nd_, s_);
+    else
+      launch_density<float, false>(pos_, vel_, nullptr, hydro_, dots_, hash_, ngb_offset_, ngb_count_, list_, nd_, s_);
+  } else {
+    launch_density<double, false>(pos_, vel_, nullptr, hydro_, dots_, hash_, ngb_offset_, ngb_count_, list_, nd_, s_);
+  }
+}
+
+template <class T>
+void Engine<T>::eos() {
+  TimerScope ts(timers_, "eos", s_);
+  const EosConst c = eos_const();
+  if constexpr (kF32) {
+    if (mixed()) {
+      launch_eos_mixed((F4*)hydro_.p, u64_, n_, c, s_);
+      return;
+    }
+  }
+  launch_eos<T>(hydro_, nd_, c, s_);
+}
+
+template <class T>
+void Engine<T>::hydro() {
+  TimerScope ts(timers_, "hydro", s_);
+  const T a = T(p_.av_alpha), b = T(p_.av_beta);
+  if constexpr (kF32) {
+    if (mixed())
+      launch_hydro<float, true>(pos_, vel_, pos_lo_, hydro_, dots_, bdots_, hydro2_, acc_, hash_, ngb_offset_,
+                                ngb_count_, list_, nd_, a, b, s_);
+    else
+      launch_hydro<float, false>(pos_, vel_, nullptr, hydro_, dots_, bdots_, hydro2_, acc_, hash_, ngb_offset_,
+                                 ngb_count_, list_, nd_, a, b, s_);
+  } else {
+    launch_hydro<double, false>(pos_, vel_, nullptr, hydro_, dots_, bdots_, hydro2_, acc_, hash_, ngb_offset_,
+                                ngb_count_, list_, nd_, a, b, s_);
+  }
+}
+
+// reaction scatter (sph_accelerations.cpp:104-115) as a gather: sort the list
+// slots by target j (stable, so each j sees its sources in ascending slot =
+// ascending i order, the reference's serial order) and let each j sum its
+// own reactions
+template <class T>
+void Engine<T>::reaction() {
+  TimerScope ts(timers_, "reaction", s_);
+  launch_reaction_pairs(ngb_offset_, ngb_count_, list_, pairs_, owner_, nd_, s_);
+  launch_radix_sort(pairs_, pairs_alt_, ngb_tot_, bits_for(nd_), sort_tmp_, s_);
+  CS_HIP(hipMemsetAsync(tr_start_, 0, tr_start_.bytes(), s_));
+  CS_HIP(hipMemsetAsync(tr_end_, 0, tr_end_.bytes(), s_));
+  launch_reaction_bounds(pairs_, ngb_tot_, nd_, tr_start_, tr_end_, s_);
+  if constexpr (kF32) {
+    if (mixed()) {
+      launch_reaction_mixed(pos_, pos_lo_, bdots_, acc_, pairs_, owner_, tr_start_, tr_end_, n_, s_);
+      return;
+    }
+  }
+  launch_reaction<T>(pos_, bdots_, acc_, pairs_, owner_, tr_start_, tr_end_, n_, s_);
+}
+
+template <class T>
+void Engine<T>::sph_accelerations() {
+  tree();
+  solve();
+  offsets();
+  find();
+  density();
+  eos();
+  hydro();
+  reaction();
+}
+
+// CUDA_gravity.cpp:4-71 + sph_grav_kernel.cu
+template <class T>
+void Engine<T>::gravity() {
+  refresh_if_dirty();
+  if constexpr (kF32) {
+    if (mixed()) {
+      launch_gravity_pack_mixed(pos64_, pos_, vel_, ghi_, glo_, gvel32_, n_, nd_, s_);
+      TimerScope ts(timers_, "gravity", s_);
+      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, nd_, nd_, p_.grav_mode, p_.grav_eps, s_);
+      return;
+    }
+  }
+  launch_gravity_pack<T>(pos_, vel_, gpos_, gvel_, n_, nd_, s_);
+  TimerScope ts(timers_, "gravity", s_);
+  launch_gravity<T>(gpos_, gvel_, grav_, nd_, nd_, p_.grav_mode, T(p_.grav_eps), exact_rsqrt_, s_);
+}
+
+template <class T>
+void Engine<T>::add_gravity() {
+  launch_add_gravity<T>(bdots_, grav_, pos_, vel_, n_, partials_, force6_, s_);
+}
+
+template <class T>
+void Engine<T>::timestep(double* out) {
+  if constexpr (kF32) {
+    if (mixed()) {
+      launch_timestep_mixed(pos_, u64_, hydro2_, bdots_, n_, p_.cn_vel, p_.cn_eth, partials_, out, s_);
+      return;
+    }
+  }
+  launch_timestep<T>(pos_, hydro_, hydro2_, bdots_, n_, p_.cn_vel, p_.cn_eth, kF32 ? 1 : 0, partials_, out, s_);
+}
+
+template <class T>
+void Engine<T>::predict() {
+  refresh_if_dirty();
+  if constexpr (kF32) {
+    if (mixed()) {
+      launch_predict_mixed(pos64_, vel64_, u64_, old64_, pos_, pos_lo_, vel_, hydro_, bdots_, grav_, dt_cur_, n_, s_);
+      return;
+    }
+  }
+  launch_predict<T>(pos_, vel_, hydro_, bdots_, grav_, old_vu_, dt_cur_, n_, s_);
+}
+
+template <class T>
+void Engine<T>::correct(const double* dt_new, const double* dt_old) {
+  if constexpr (kF32) {
+    if (mixed()) {
+      launch_correct_mixed(vel64_, u64_, old64_, vel_, hydro_, bdots_, dt_new, dt_old, n_, s_);
+      return;
+    }
+  }
+  launch_correct<T>(vel_, hydro_, bdots_, old_vu_, dt_new, dt_old, n_, s_);
+}
+
+template <class T>
+void Engine<T>::stats_pass() {
+  if constexpr (kF32) {
+    if (mixed()) {
+      launch_stats_mixed(pos64_, vel64_, u64_, grav_, n_, partials_, stats6_, s_);
+      return;
+    }
+  }
+  launch_stats<T>(pos_, vel_, hydro_, grav_, n_, partials_, stats6_, s_);
+}
+
+// sort_bodies.cpp:119-186 on the GPU: 3*bits-bit Peano-Hilbert keys sorted
+// stably (low 32 bits, then the high bits), then the seven state arrays
+// gathered through the permutation
+template <class T>
+void Engine<T>::sort_bodies() {
+  refresh_if_dirty();
+  TimerScope ts(timers_, "ph_sort", s_);
+  launch_ph_keys<T>(pos_, n_, p_.ph_bits, key_hi_, key_lo_, scan_tmp_, s_);
+  // (key_lo, i) -> sort -> (key_hi[perm], perm) -> sort
+  KV* kv = pairs_.p;
+  KV* alt = pairs_alt_.p;
+  launch_iota_kv(kv, n_, s_);
+  launch_set_keys(kv, key_lo_, n_, s_);
+  launch_radix_sort(kv, alt, n_, 32, sort_tmp_, s_);
+  // second pass keys: key_hi of the permuted order
+  launch_gather<uint32_t>(key_hi_, key_lo_, kv, n_, s_);
+  launch_set_keys(kv, key_lo_, n_, s_);
+  const int hi_bits = std::max(0, 3 * p_.ph_bits - 32);
+  launch_radix_sort(kv, alt, n_, hi_bits, sort_tmp_, s_);
+  auto permute = [&](DBuf<V4<T>>& a) {
+    launch_gather<V4<T>>(a, tmp4_, kv, n_, s_);
+    CS_HIP(hipMemcpyAsync(a, tmp4_, (size_t)n_ * sizeof(V4<T>), hipMemcpyDeviceToDevice, s_));
+  };
+  permute(pos_);
+  permute(vel_);
+  permute(bdots_);
+  permute(hydro_);
+  permute(dots_);
+  permute(hydro2_);
+  permute(grav_);
+  if constexpr (kF32) {
+    if (mixed()) {
+      for (DBuf<D4>* a : {&pos64_, &vel64_}) {
+        launch_gather<D4>(*a, old64_, kv, n_, s_);
+        CS_HIP(hipMemcpyAsync(a->p, old64_, (size_t)n_ * sizeof(D4), hipMemcpyDeviceToDevice, s_));
+      }
+      launch_gather<double>(u64_, (double*)tmp4_.p, kv, n_, s_);
+      CS_HIP(hipMemcpyAsync(u64_, tmp4_, (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+      launch_refresh_mixed(pos64_, vel64_, u64_, (F4*)pos_.p, pos_lo_, (F4*)vel_.p, (F4*)hydro_.p, n_, s_);
+    }
+  }
+}
+
+template <class T>
+void Engine<T>::init() {
+  sph_accelerations();
+  gravity();
+  add_gravity();
+  stats_pass();
+  timestep(dt_cur_);
+}
+
+// step.cpp:13-147
+template <class T>
+void Engine<T>::step() {
+  ++iteration_;
+  if (p_.ph_period > 0 && iteration_ % p_.ph_period == 0) sort_bodies();
+  predict();
+  sph_accelerations();   // reads back dt_ with the positions
+  gravity();
+  add_gravity();
+  global_time_ += dt_;
+  stats_pass();
+  timestep(dt_next_);
+  correct(dt_next_, dt_cur_);
+  CS_HIP(hipMemcpyAsync(dt_cur_, dt_next_, 8, hipMemcpyDeviceToDevice, s_));
+}
+
+template <class T>
+void Engine<T>::pass(int id) {
+  switch (id) {
+    case SPH_TREE: tree(); break;
+    case SPH_SOLVE: solve(); break;
+    case SPH_OFFSETS: offsets(); break;
+    case SPH_FIND: find(); break;
+    case SPH_DENSITY: density(); break;
+    case SPH_EOS: eos(); break;
+    case SPH_HYDRO: hydro(); break;
+    case SPH_REACTION: reaction(); break;
+    case SPH_GRAVITY: gravity(); break;
+    case SPH_ADD_GRAVITY: add_gravity(); break;
+    case SPH_TIMESTEP: timestep(dt_cur_); break;
+    case SPH_PREDICT: predict(); break;
+    case SPH_CORRECT: correct(dt_cur_, dt_cur_); break;
+    case SPH_STATS: stats_pass(); break;
+    case SPH_PH_SORT: sort_bodies(); break;
+    case SPH_SPH_ACC: sph_accelerations(); break;
+    case SPH_INIT: init(); break;
+    case SPH_STEP: step(); break;
+    default: throw std::invalid_argument("unknown pass id");
+  }
+}
+
+EngineBase* make_engine(int form, const sph_params& p, int device) {
+  if (form == SPH_FORM_PLAIN || form == SPH_FORM_MIXED) return new Engine<float>(form, p, device);
+  if (form == SPH_FORM_FP64) return new Engine<double>(form, p, device);
+  throw std::invalid_argument("unknown form");
+}
+
+}  // namespace cudasph

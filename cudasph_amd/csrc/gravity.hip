@@ -1,0 +1,256 @@
+// Softened direct-N gravity (sph_grav_kernel.cu:14-195).
+//
+// Layout: one lane per i-particle, 256 i per block.  Every lane of a
+// wavefront walks the same j, so the j record's address is wave-uniform and
+// the compiler reads it with scalar loads into SGPRs; the pair arithmetic
+// then takes its j operands straight from SGPRs and the loop body is pure
+// VALU work (no LDS staging, no per-j vector loads).  j runs in ascending
+// order over all records, the reference's summation order.
+//
+// Variants (template parameters):
+//   SOFT  0: M4 spline at the SPH range, half from each side (reference)
+//         1: Plummer eps
+//   ACC   0: per-pair double-float (dreal) accumulation (reference, dreal.h)
+//         1: blocked compensation -- a plain FP32 partial over each tile of
+//            kTile j, folded into a two-sum (hi,lo) total (mixed form)
+//         2: plain accumulation in T (FP64 form)
+//   EXACT 1: 1/sqrt correctly rounded (bit-exact with the CPU restatement);
+//         0: hardware reciprocal square root (the reference uses rsqrtf)
+//   HILO  positions carry low words (mixed form): r_ij = (hi_i-hi_j)+(lo_i-lo_j)
+#include "kernels.h"
+#include "sph_math.h"
+
+namespace cudasph {
+
+struct DR {
+  float hi, lo;
+};
+// two-sum subtraction with renormalisation (Knuth; the reference's dsub,
+// dreal.h:43-52)
+__device__ __forceinline__ void dr_sub(DR& x, float y) {
+  const float t1 = x.hi - y;
+  const float e = t1 - x.hi;
+  const float t2 = ((-y - e) + (x.hi - (t1 - e))) + x.lo;
+  const float s = t1 + t2;
+  x.lo = t2 - (s - t1);
+  x.hi = s;
+}
+__device__ __forceinline__ void dr_add(DR& x, float y) { dr_sub(x, -y); }
+
+template <class T, int ACC>
+struct GAcc;
+template <>
+struct GAcc<float, 0> {                      // per-pair dreal
+  DR v{0.f, 0.f};
+  __device__ __forceinline__ void sub(float y) { dr_sub(v, y); }
+  __device__ __forceinline__ void tile_end() {}
+  __device__ __forceinline__ float get() const { return v.hi; }
+};
+template <>
+struct GAcc<float, 1> {                      // blocked compensation
+  DR v{0.f, 0.f};
+  float part = 0.f;
+  __device__ __forceinline__ void sub(float y) { part -= y; }
+  __device__ __forceinline__ void fsub(float a, float b) { part = __builtin_fmaf(-a, b, part); }
+  __device__ __forceinline__ void tile_end() {
+    dr_add(v, part);
+    part = 0.f;
+  }
+  __device__ __forceinline__ float get() const { return v.hi + v.lo; }
+};
+template <>
+struct GAcc<double, 2> {
+  double v = 0.0;
+  __device__ __forceinline__ void sub(double y) { v -= y; }
+  __device__ __forceinline__ void tile_end() {}
+  __device__ __forceinline__ double get() const { return v; }
+};
+
+template <bool EXACT>
+__device__ __forceinline__ float inv_sqrt(float x) {
+  if constexpr (EXACT) return 1.0f / __builtin_sqrtf(x);
+  else return __builtin_amdgcn_rsqf(x);
+}
+template <bool EXACT>
+__device__ __forceinline__ double inv_sqrt(double x) {
+  return 1.0 / __builtin_sqrt(x);
+}
+
+// spline-softened pair (sph_grav_kernel.cu:14-111); pj = (x,y,z,m),
+// vi.w / vj.w = range^2
+template <class T, bool EXACT, class A>
+__device__ __forceinline__ void pair_spline(A* a, T rx, T ry, T rz, T ri2, T mj, T rj2) {
+  const T d2 = rx * rx + ry * ry + rz * rz;
+  const T inv = inv_sqrt<EXACT>(d2);
+  const T inv3 = inv * inv * inv;
+  T s3 = mj * inv3;
+  if (d2 >= ri2 && d2 >= rj2) {
+    a[0].sub(rx * s3);
+    a[1].sub(ry * s3);
+    a[2].sub(rz * s3);
+    a[3].sub(mj * inv);
+    return;
+  }
+  s3 = T(0.5f) * s3;
+  if (d2 < ri2) {
+    const T u2 = d2 / ri2, h = dsqrt(ri2), u = dsqrt(u2), u3 = u2 * u;
+    T gacc, gpot;
+    if (u < T(0.5)) {
+      gacc = T(32.0 / 3.0) + u2 * (T(32) * u - T(38.4));
+      gpot = T(-2.8) + u2 * (T(16.0 / 3.0) + u2 * (T(6.4) * u - T(9.6)));
+    } else {
+      gacc = T(64.0 / 3.0) - T(48) * u + T(38.4) * u2 - T(32.0 / 3.0) * u3 - T(1.0 / 15.0) / u3;
+      gpot = T(-3.2) + T(1.0 / 15.0) / u + u2 * (T(32.0 / 3.0) + u * (T(-16) + u * (T(9.6) - T(32.0 / 15.0) * u)));
+    }
+    gacc = T(0.5 * (double)mj * (double)gacc / (double)(ri2 * h));
+    gpot = -gpot;
+    a[0].sub(rx * gacc);
+    a[1].sub(ry * gacc);
+    a[2].sub(rz * gacc);
+    a[3].sub(mj * gpot / h);
+  } else {
+    a[0].sub(rx * s3);
+    a[1].sub(ry * s3);
+    a[2].sub(rz * s3);
+    a[3].sub(mj * inv);
+  }
+  if (d2 < rj2) {
+    const T u2 = d2 / rj2, h = dsqrt(rj2), u = dsqrt(u2), u3 = u2 * u;
+    T gacc;
+    if (u < T(0.5)) gacc = T(32.0 / 3.0) + u2 * (T(32) * u - T(38.4));
+    else gacc = T(64.0 / 3.0) - T(48) * u + T(38.4) * u2 - T(32.0 / 3.0) * u3 - T(1.0 / 15.0) / u3;
+    gacc = T(0.5 * (double)mj * (double)gacc / (double)(rj2 * h));
+    a[0].sub(rx * gacc);
+    a[1].sub(ry * gacc);
+    a[2].sub(rz * gacc);
+  } else {
+    a[0].sub(rx * s3);
+    a[1].sub(ry * s3);
+    a[2].sub(rz * s3);
+  }
+}
+
+// Plummer pair: a -= m r (r^2+eps^2)^-3/2, phi -= m (r^2+eps^2)^-1/2
+template <class T, bool EXACT, class A>
+__device__ __forceinline__ void pair_plummer(A* a, T rx, T ry, T rz, T mj, T eps2) {
+  const T d2 = rx * rx + ry * ry + rz * rz + eps2;
+  const T inv = inv_sqrt<EXACT>(d2);
+  const T inv3 = inv * inv * inv;
+  const T s3 = mj * inv3;
+  a[0].sub(rx * s3);
+  a[1].sub(ry * s3);
+  a[2].sub(rz * s3);
+  a[3].sub(mj * inv);
+}
+
+constexpr int kGravBlock = 256;
+constexpr int kTile = 64;
+
+// reference-layout kernel (plain and FP64 forms): gpos = (x,y,z,m),
+// gvel.w = range^2
+template <class T, int SOFT, int ACC, bool EXACT>
+__global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict__ gpos,
+                                                        const V4<T>* __restrict__ gvel, V4<T>* __restrict__ out,
+                                                        int n_i, int n_j, T eps2) {
+  const int i = blockIdx.x * kGravBlock + threadIdx.x;
+  const int ii = i < n_i ? i : n_i - 1;
+  const V4<T> pi = gpos[ii];
+  const T ri2 = SOFT == 0 ? gvel[ii].w : T(0);
+  GAcc<T, ACC> a[4];
+  for (int j = 0; j < n_j; ++j) {
+    const V4<T> pj = gpos[j];
+    const T rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    if constexpr (SOFT == 0) pair_spline<T, EXACT>(a, rx, ry, rz, ri2, pj.w, gvel[j].w);
+    else pair_plummer<T, EXACT>(a, rx, ry, rz, pj.w, eps2);
+  }
+  if (i < n_i) out[i] = {a[0].get(), a[1].get(), a[2].get(), a[3].get()};
+}
+
+// mixed form, Plummer: split positions, FMA pair arithmetic, blocked
+// compensated sums.  hi = (x,y,z,m) rounded, lo = (x-hi, y-hi, z-hi, 0).
+__global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* __restrict__ hi,
+                                                                       const F4* __restrict__ lo,
+                                                                       F4* __restrict__ out, int n_i, int n_j,
+                                                                       float eps2) {
+  const int i = blockIdx.x * kGravBlock + threadIdx.x;
+  const int ii = i < n_i ? i : n_i - 1;
+  const F4 pi = hi[ii], li = lo[ii];
+  GAcc<float, 1> a[4];
+  int j = 0;
+  for (int j0 = 0; j0 < n_j; j0 += kTile) {
+    const int j1 = min(j0 + kTile, n_j);
+    for (j = j0; j < j1; ++j) {
+      const F4 pj = hi[j], lj = lo[j];
+      const float rx = (pi.x - pj.x) + (li.x - lj.x);
+      const float ry = (pi.y - pj.y) + (li.y - lj.y);
+      const float rz = (pi.z - pj.z) + (li.z - lj.z);
+      const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+      const float inv = __builtin_amdgcn_rsqf(d2);
+      const float mi = pj.w * inv;
+      const float s3 = mi * (inv * inv);
+      a[0].fsub(rx, s3);
+      a[1].fsub(ry, s3);
+      a[2].fsub(rz, s3);
+      a[3].sub(mi);
+    }
+    for (int k = 0; k < 4; ++k) a[k].tile_end();
+  }
+  if (i < n_i) out[i] = {a[0].get(), a[1].get(), a[2].get(), a[3].get()};
+}
+
+// mixed form, spline softening: same accumulation and split positions
+__global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_spline(const F4* __restrict__ hi,
+                                                                      const F4* __restrict__ lo,
+                                                                      const F4* __restrict__ gvel,
+                                                                      F4* __restrict__ out, int n_i, int n_j) {
+  const int i = blockIdx.x * kGravBlock + threadIdx.x;
+  const int ii = i < n_i ? i : n_i - 1;
+  const F4 pi = hi[ii], li = lo[ii];
+  const float ri2 = gvel[ii].w;
+  GAcc<float, 1> a[4];
+  for (int j0 = 0; j0 < n_j; j0 += kTile) {
+    const int j1 = min(j0 + kTile, n_j);
+    for (int j = j0; j < j1; ++j) {
+      const F4 pj = hi[j], lj = lo[j];
+      const float rx = (pi.x - pj.x) + (li.x - lj.x);
+      const float ry = (pi.y - pj.y) + (li.y - lj.y);
+      const float rz = (pi.z - pj.z) + (li.z - lj.z);
+      pair_spline<float, false>(a, rx, ry, rz, ri2, pj.w, gvel[j].w);
+    }
+    for (int k = 0; k < 4; ++k) a[k].tile_end();
+  }
+  if (i < n_i) out[i] = {a[0].get(), a[1].get(), a[2].get(), a[3].get()};
+}
+
+template <class T>
+void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, int n_j, int soft, T eps,
+                    int exact, hipStream_t s) {
+  const dim3 grid(ceil_div(n_i, kGravBlock)), block(kGravBlock);
+  const T eps2 = eps * eps;
+  if constexpr (sizeof(T) == 4) {
+    if (soft == 0) {
+      if (exact) hipLaunchKernelGGL((k_gravity<float, 0, 0, true>), grid, block, 0, s, gpos, gvel, out, n_i, n_j, eps2);
+      else hipLaunchKernelGGL((k_gravity<float, 0, 0, false>), grid, block, 0, s, gpos, gvel, out, n_i, n_j, eps2);
+    } else {
+      if (exact) hipLaunchKernelGGL((k_gravity<float, 1, 0, true>), grid, block, 0, s, gpos, gvel, out, n_i, n_j, eps2);
+      else hipLaunchKernelGGL((k_gravity<float, 1, 0, false>), grid, block, 0, s, gpos, gvel, out, n_i, n_j, eps2);
+    }
+  } else {
+    if (soft == 0) hipLaunchKernelGGL((k_gravity<double, 0, 2, true>), grid, block, 0, s, gpos, gvel, out, n_i, n_j, eps2);
+    else hipLaunchKernelGGL((k_gravity<double, 1, 2, true>), grid, block, 0, s, gpos, gvel, out, n_i, n_j, eps2);
+  }
+  CS_CHECK_LAUNCH();
+}
+
+void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, int n_i, int n_j, int soft,
+                          float eps, hipStream_t s) {
+  const dim3 grid(ceil_div(n_i, kGravBlock)), block(kGravBlock);
+  if (soft == 0) hipLaunchKernelGGL(k_gravity_mixed_spline, grid, block, 0, s, hi, lo, gvel, out, n_i, n_j);
+  else hipLaunchKernelGGL(k_gravity_mixed_plummer, grid, block, 0, s, hi, lo, out, n_i, n_j, eps * eps);
+  CS_CHECK_LAUNCH();
+}
+
+template void launch_gravity<float>(const F4*, const F4*, F4*, int, int, int, float, int, hipStream_t);
+template void launch_gravity<double>(const D4*, const D4*, D4*, int, int, int, double, int, hipStream_t);
+
+}  // namespace cudasph

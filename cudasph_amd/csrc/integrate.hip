@@ -1,0 +1,497 @@
+// Integration and bookkeeping kernels: gravity packing, predictor and
+// corrector (step.cpp:37-61, 128-134), adding gravity with the force and
+// torque diagnostics (step.cpp:97-118), the timestep minimum
+// (compute_timestep.cpp:3-25) and the energy / angular momentum sums
+// (system_statistics.cpp:4-84).
+//
+// The plain form keeps the reference's types: FP32 state, a double dt, and a
+// float half step dth.  The mixed form integrates an FP64 copy of x, v and u
+// and refreshes the FP32 arrays the SPH passes read (positions as hi + lo
+// words).  Reductions are two-level (per-block partials, then one block in
+// fixed order) and accumulate in FP64.
+#include "kernels.h"
+#include "sph_math.h"
+
+namespace cudasph {
+
+constexpr int kRedBlock = 256;
+
+// CUDA_gravity.cpp:36-56: (x,y,z,m) and (vx,vy,vz,range^2); padding records
+// copy real positions scaled by 1.1 with zero mass
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gravity_pack(const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel,
+                                                          V4<T>* __restrict__ gpos, V4<T>* __restrict__ gvel, int n,
+                                                          int n_dev) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n_dev) return;
+  if (i < n) {
+    const V4<T> p = pos[i], v = vel[i];
+    gpos[i] = {p.x, p.y, p.z, v.w};
+    gvel[i] = {v.x, v.y, v.z, p.w * p.w};
+  } else {
+    const V4<T> p = pos[(i - n) % n];
+    gpos[i] = {T((double)p.x * 1.1), T((double)p.y * 1.1), T((double)p.z * 1.1), T(0)};
+    gvel[i] = {T(0), T(0), T(0), T(0)};
+  }
+}
+
+template <class T>
+void launch_gravity_pack(const V4<T>* pos, const V4<T>* vel, V4<T>* gpos, V4<T>* gvel, int n, int n_dev,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_gravity_pack<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, vel, gpos, gvel, n,
+                     n_dev);
+  CS_CHECK_LAUNCH();
+}
+template void launch_gravity_pack<float>(const F4*, const F4*, F4*, F4*, int, int, hipStream_t);
+template void launch_gravity_pack<double>(const D4*, const D4*, D4*, D4*, int, int, hipStream_t);
+
+__global__ __launch_bounds__(kBlock) void k_gravity_pack_mixed(const D4* __restrict__ pos64, const F4* __restrict__ pos,
+                                                                const F4* __restrict__ vel, F4* __restrict__ hi,
+                                                                F4* __restrict__ lo, F4* __restrict__ gvel, int n,
+                                                                int n_dev) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n_dev) return;
+  if (i < n) {
+    const D4 x = pos64[i];
+    const F4 p = pos[i];
+    const float hx = (float)x.x, hy = (float)x.y, hz = (float)x.z;
+    hi[i] = {hx, hy, hz, vel[i].w};
+    lo[i] = {(float)(x.x - hx), (float)(x.y - hy), (float)(x.z - hz), 0.f};
+    gvel[i] = {0.f, 0.f, 0.f, p.w * p.w};
+  } else {
+    const D4 x = pos64[(i - n) % n];
+    hi[i] = {(float)(x.x * 1.1), (float)(x.y * 1.1), (float)(x.z * 1.1), 0.f};
+    lo[i] = {0.f, 0.f, 0.f, 0.f};
+    gvel[i] = {0.f, 0.f, 0.f, 0.f};
+  }
+}
+
+void launch_gravity_pack_mixed(const D4* pos64, const F4* pos, const F4* vel, F4* hi, F4* lo, F4* gvel, int n,
+                               int n_dev, hipStream_t s) {
+  hipLaunchKernelGGL(k_gravity_pack_mixed, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos64, pos, vel, hi,
+                     lo, gvel, n, n_dev);
+  CS_CHECK_LAUNCH();
+}
+
+// step.cpp:37-61.  old_vu keeps (vx, vy, vz, u) for the corrector.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_predict(V4<T>* __restrict__ pos, V4<T>* __restrict__ vel,
+                                                     V4<T>* __restrict__ hydro, V4<T>* __restrict__ bdots,
+                                                     V4<T>* __restrict__ grav, V4<T>* __restrict__ old_vu,
+                                                     const double* __restrict__ dt_p, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double dt = *dt_p;
+  const T dth = T(0.5 * dt);
+  V4<T> p = pos[i], v = vel[i], h = hydro[i], d = bdots[i];
+  p.x = T((double)p.x + (double)v.x * dt);
+  p.y = T((double)p.y + (double)v.y * dt);
+  p.z = T((double)p.z + (double)v.z * dt);
+  old_vu[i] = {v.x, v.y, v.z, h.z};
+  v.x += d.x * dth;
+  v.y += d.y * dth;
+  v.z += d.z * dth;
+  h.z += d.w * dth;
+  d.x = d.y = d.z = T(0);
+  pos[i] = p;
+  vel[i] = v;
+  hydro[i] = h;
+  bdots[i] = d;
+  V4<T> g = grav[i];
+  g.x = g.y = g.z = T(0);
+  grav[i] = g;
+}
+
+template <class T>
+void launch_predict(V4<T>* pos, V4<T>* vel, V4<T>* hydro, V4<T>* bdots, V4<T>* grav, V4<T>* old_vu,
+                    const double* dt, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_predict<T>, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, pos, vel, hydro, bdots, grav,
+                     old_vu, dt, n);
+  CS_CHECK_LAUNCH();
+}
+template void launch_predict<float>(F4*, F4*, F4*, F4*, F4*, F4*, const double*, int, hipStream_t);
+template void launch_predict<double>(D4*, D4*, D4*, D4*, D4*, D4*, const double*, int, hipStream_t);
+
+// step.cpp:128-134: v = v_old + a (dt_new/2 + dth_old)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_correct(V4<T>* __restrict__ vel, V4<T>* __restrict__ hydro,
+                                                     const V4<T>* __restrict__ bdots, const V4<T>* __restrict__ old_vu,
+                                                     const double* __restrict__ dt_new,
+                                                     const double* __restrict__ dt_old, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double f = 0.5 * (*dt_new) + (double)T(0.5 * (*dt_old));
+  const V4<T> o = old_vu[i], d = bdots[i];
+  V4<T> v = vel[i];
+  v.x = T((double)o.x + (double)d.x * f);
+  v.y = T((double)o.y + (double)d.y * f);
+  v.z = T((double)o.z + (double)d.z * f);
+  vel[i] = v;
+  hydro[i].z = T((double)o.w + (double)d.w * f);
+}
+
+template <class T>
+void launch_correct(V4<T>* vel, V4<T>* hydro, const V4<T>* bdots, const V4<T>* old_vu, const double* dt_new,
+                    const double* dt_old, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_correct<T>, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, vel, hydro, bdots, old_vu, dt_new,
+                     dt_old, n);
+  CS_CHECK_LAUNCH();
+}
+template void launch_correct<float>(F4*, F4*, const F4*, const F4*, const double*, const double*, int, hipStream_t);
+template void launch_correct<double>(D4*, D4*, const D4*, const D4*, const double*, const double*, int,
+                                     hipStream_t);
+
+// FP32 views of the FP64 state: positions as hi + lo words
+__device__ __forceinline__ void refresh_one(const D4& x, const D4& v, double u, F4& pos, F4& lo, F4& vel,
+                                            F4& hydro) {
+  const float hx = (float)x.x, hy = (float)x.y, hz = (float)x.z;
+  pos.x = hx;
+  pos.y = hy;
+  pos.z = hz;
+  lo = {(float)(x.x - hx), (float)(x.y - hy), (float)(x.z - hz), 0.f};
+  vel.x = (float)v.x;
+  vel.y = (float)v.y;
+  vel.z = (float)v.z;
+  hydro.z = (float)u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_predict_mixed(D4* __restrict__ pos64, D4* __restrict__ vel64,
+                                                           double* __restrict__ u64, D4* __restrict__ old64,
+                                                           F4* __restrict__ pos, F4* __restrict__ pos_lo,
+                                                           F4* __restrict__ vel, F4* __restrict__ hydro,
+                                                           F4* __restrict__ bdots, F4* __restrict__ grav,
+                                                           const double* __restrict__ dt_p, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double dt = *dt_p, dth = 0.5 * dt;
+  D4 x = pos64[i], v = vel64[i];
+  double u = u64[i];
+  F4 d = bdots[i];
+  x.x += v.x * dt;
+  x.y += v.y * dt;
+  x.z += v.z * dt;
+  old64[i] = {v.x, v.y, v.z, u};
+  v.x += (double)d.x * dth;
+  v.y += (double)d.y * dth;
+  v.z += (double)d.z * dth;
+  u += (double)d.w * dth;
+  pos64[i] = x;
+  vel64[i] = v;
+  u64[i] = u;
+  F4 p = pos[i], vv = vel[i], h = hydro[i], lo;
+  refresh_one(x, v, u, p, lo, vv, h);
+  pos[i] = p;
+  pos_lo[i] = lo;
+  vel[i] = vv;
+  hydro[i] = h;
+  d.x = d.y = d.z = 0.f;
+  bdots[i] = d;
+  F4 g = grav[i];
+  g.x = g.y = g.z = 0.f;
+  grav[i] = g;
+}
+
+void launch_predict_mixed(D4* pos64, D4* vel64, double* u64, D4* old64, F4* pos, F4* pos_lo, F4* vel, F4* hydro,
+                          F4* bdots, F4* grav, const double* dt, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_predict_mixed, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, pos64, vel64, u64, old64, pos,
+                     pos_lo, vel, hydro, bdots, grav, dt, n);
+  CS_CHECK_LAUNCH();
+}
+
+__global__ __launch_bounds__(kBlock) void k_correct_mixed(D4* __restrict__ vel64, double* __restrict__ u64,
+                                                           const D4* __restrict__ old64, F4* __restrict__ vel,
+                                                           F4* __restrict__ hydro, const F4* __restrict__ bdots,
+                                                           const double* __restrict__ dt_new,
+                                                           const double* __restrict__ dt_old, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double f = 0.5 * (*dt_new) + 0.5 * (*dt_old);
+  const D4 o = old64[i];
+  const F4 d = bdots[i];
+  D4 v = vel64[i];
+  v.x = o.x + (double)d.x * f;
+  v.y = o.y + (double)d.y * f;
+  v.z = o.z + (double)d.z * f;
+  const double u = o.w + (double)d.w * f;
+  vel64[i] = v;
+  u64[i] = u;
+  F4 vv = vel[i];
+  vv.x = (float)v.x;
+  vv.y = (float)v.y;
+  vv.z = (float)v.z;
+  vel[i] = vv;
+  hydro[i].z = (float)u;
+}
+
+void launch_correct_mixed(D4* vel64, double* u64, const D4* old64, F4* vel, F4* hydro, const F4* bdots,
+                          const double* dt_new, const double* dt_old, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_correct_mixed, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, vel64, u64, old64, vel, hydro,
+                     bdots, dt_new, dt_old, n);
+  CS_CHECK_LAUNCH();
+}
+
+__global__ __launch_bounds__(kBlock) void k_refresh_mixed(const D4* __restrict__ pos64, const D4* __restrict__ vel64,
+                                                           const double* __restrict__ u64, F4* __restrict__ pos,
+                                                           F4* __restrict__ pos_lo, F4* __restrict__ vel,
+                                                           F4* __restrict__ hydro, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  F4 p = pos[i], vv = vel[i], h = hydro[i], lo;
+  refresh_one(pos64[i], vel64[i], u64[i], p, lo, vv, h);
+  vv.w = (float)vel64[i].w;
+  pos[i] = p;
+  pos_lo[i] = lo;
+  vel[i] = vv;
+  hydro[i] = h;
+}
+
+void launch_refresh_mixed(const D4* pos64, const D4* vel64, const double* u64, F4* pos, F4* pos_lo, F4* vel,
+                          F4* hydro, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_refresh_mixed, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, pos64, vel64, u64, pos, pos_lo,
+                     vel, hydro, n);
+  CS_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// reductions: each block reduces 6 doubles into partials[6*block + c]; one
+// block then sums the partials in index order.
+__device__ __forceinline__ void block_sum6(double v[6], double* partials) {
+  __shared__ double s[6][kRedBlock];
+  for (int c = 0; c < 6; ++c) s[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int off = kRedBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int c = 0; c < 6; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) partials[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_final_sum6(const double* __restrict__ partials, int nb,
+                                                           double* __restrict__ out6) {
+  __shared__ double s[6][kRedBlock];
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += kRedBlock)
+    for (int c = 0; c < 6; ++c) v[c] += partials[6 * b + c];
+  for (int c = 0; c < 6; ++c) s[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int off = kRedBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int c = 0; c < 6; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out6[threadIdx.x] = s[threadIdx.x][0];
+}
+
+long reduce_partials(int n) { return 6L * ceil_div(n, kRedBlock) + 6; }
+
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void k_add_gravity(V4<T>* __restrict__ bdots, const V4<T>* __restrict__ grav,
+                                                            const V4<T>* __restrict__ pos,
+                                                            const V4<T>* __restrict__ vel, int n,
+                                                            double* __restrict__ partials) {
+  const int i = blockIdx.x * kRedBlock + threadIdx.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    V4<T> d = bdots[i];
+    const V4<T> g = grav[i], p = pos[i];
+    d.x += g.x;
+    d.y += g.y;
+    d.z += g.z;
+    bdots[i] = d;
+    const T m = vel[i].w;
+    v[0] = m * d.x;
+    v[1] = m * d.y;
+    v[2] = m * d.z;
+    v[3] = m * (p.y * d.z - p.z * d.y);
+    v[4] = m * (p.z * d.x - p.x * d.z);
+    v[5] = m * (p.x * d.y - p.y * d.x);
+  }
+  block_sum6(v, partials);
+}
+
+template <class T>
+void launch_add_gravity(V4<T>* bdots, const V4<T>* grav, const V4<T>* pos, const V4<T>* vel, int n,
+                        double* partials, double* out6, hipStream_t s) {
+  const int nb = ceil_div(n, kRedBlock);
+  hipLaunchKernelGGL(k_add_gravity<T>, dim3(nb), dim3(kRedBlock), 0, s, bdots, grav, pos, vel, n, partials);
+  CS_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_final_sum6, dim3(1), dim3(kRedBlock), 0, s, partials, nb, out6);
+  CS_CHECK_LAUNCH();
+}
+template void launch_add_gravity<float>(F4*, const F4*, const F4*, const F4*, int, double*, double*, hipStream_t);
+template void launch_add_gravity<double>(D4*, const D4*, const D4*, const D4*, int, double*, double*, hipStream_t);
+
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void k_stats(const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel,
+                                                      const V4<T>* __restrict__ hydro, const V4<T>* __restrict__ grav,
+                                                      int n, double* __restrict__ partials) {
+  const int i = blockIdx.x * kRedBlock + threadIdx.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const V4<T> p = pos[i], u = vel[i];
+    const double m = u.w;
+    v[0] = 0.5 * m * ((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
+    v[1] = m * (double)hydro[i].z;
+    v[2] = 0.5 * m * (double)grav[i].w;
+    v[3] = m * ((double)p.y * u.z - (double)p.z * u.y);
+    v[4] = m * ((double)p.z * u.x - (double)p.x * u.z);
+    v[5] = m * ((double)p.x * u.y - (double)p.y * u.x);
+  }
+  block_sum6(v, partials);
+}
+
+template <class T>
+void launch_stats(const V4<T>* pos, const V4<T>* vel, const V4<T>* hydro, const V4<T>* grav, int n,
+                  double* partials, double* out6, hipStream_t s) {
+  const int nb = ceil_div(n, kRedBlock);
+  hipLaunchKernelGGL(k_stats<T>, dim3(nb), dim3(kRedBlock), 0, s, pos, vel, hydro, grav, n, partials);
+  CS_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_final_sum6, dim3(1), dim3(kRedBlock), 0, s, partials, nb, out6);
+  CS_CHECK_LAUNCH();
+}
+template void launch_stats<float>(const F4*, const F4*, const F4*, const F4*, int, double*, double*, hipStream_t);
+template void launch_stats<double>(const D4*, const D4*, const D4*, const D4*, int, double*, double*, hipStream_t);
+
+__global__ __launch_bounds__(kRedBlock) void k_stats_mixed(const D4* __restrict__ pos64, const D4* __restrict__ vel64,
+                                                            const double* __restrict__ u64,
+                                                            const F4* __restrict__ grav, int n,
+                                                            double* __restrict__ partials) {
+  const int i = blockIdx.x * kRedBlock + threadIdx.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const D4 p = pos64[i], u = vel64[i];
+    const double m = u.w;
+    v[0] = 0.5 * m * (u.x * u.x + u.y * u.y + u.z * u.z);
+    v[1] = m * u64[i];
+    v[2] = 0.5 * m * (double)grav[i].w;
+    v[3] = m * (p.y * u.z - p.z * u.y);
+    v[4] = m * (p.z * u.x - p.x * u.z);
+    v[5] = m * (p.x * u.y - p.y * u.x);
+  }
+  block_sum6(v, partials);
+}
+
+void launch_stats_mixed(const D4* pos64, const D4* vel64, const double* u64, const F4* grav, int n,
+                        double* partials, double* out6, hipStream_t s) {
+  const int nb = ceil_div(n, kRedBlock);
+  hipLaunchKernelGGL(k_stats_mixed, dim3(nb), dim3(kRedBlock), 0, s, pos64, vel64, u64, grav, n, partials);
+  CS_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_final_sum6, dim3(1), dim3(kRedBlock), 0, s, partials, nb, out6);
+  CS_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// timestep: per-block minimum, then one block's minimum (exact in any order)
+__device__ __forceinline__ void block_min(double v, double* out) {
+  __shared__ double s[kRedBlock];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = kRedBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void k_timestep(const V4<T>* __restrict__ pos, const V4<T>* __restrict__ hydro,
+                                                         const V4<T>* __restrict__ hydro2,
+                                                         const V4<T>* __restrict__ bdots, int n, T cn_vel, T cn_eth,
+                                                         double* __restrict__ partials) {
+  const int i = blockIdx.x * kRedBlock + threadIdx.x;
+  double v = 1.0e30;
+  if (i < n) {
+    const T vsig = hydro2[i].x;
+    const double dt_vel = cn_vel * dabs(pos[i].w / vsig);
+    const double dt_eth = cn_eth * dabs(hydro[i].z / bdots[i].w);
+    v = 1.0 / (1.0 / dt_vel + 1.0 / dt_eth);
+    if (!(v < 1.0e30)) v = 1.0e30;     // NaN and +inf leave the running minimum unchanged
+  }
+  block_min(v, partials + blockIdx.x);
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_timestep_mixed(const F4* __restrict__ pos,
+                                                               const double* __restrict__ u64,
+                                                               const F4* __restrict__ hydro2,
+                                                               const F4* __restrict__ bdots, int n, float cn_vel,
+                                                               float cn_eth, double* __restrict__ partials) {
+  const int i = blockIdx.x * kRedBlock + threadIdx.x;
+  double v = 1.0e30;
+  if (i < n) {
+    const double dt_vel = (double)cn_vel * fabs((double)pos[i].w / (double)hydro2[i].x);
+    const double dt_eth = (double)cn_eth * fabs(u64[i] / (double)bdots[i].w);
+    v = 1.0 / (1.0 / dt_vel + 1.0 / dt_eth);
+    if (!(v < 1.0e30)) v = 1.0e30;
+  }
+  block_min(v, partials + blockIdx.x);
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_final_min(const double* __restrict__ partials, int nb,
+                                                          int round_float, double* __restrict__ out) {
+  double v = 1.0e30;
+  for (int b = threadIdx.x; b < nb; b += kRedBlock) v = fmin(v, partials[b]);
+  __shared__ double s[kRedBlock];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = kRedBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = round_float ? (double)(float)s[0] : s[0];
+}
+
+template <class T>
+void launch_timestep(const V4<T>* pos, const V4<T>* hydro, const V4<T>* hydro2, const V4<T>* bdots, int n,
+                     float cn_vel, float cn_eth, int round_float, double* partials, double* dt_out,
+                     hipStream_t s) {
+  const int nb = ceil_div(n, kRedBlock);
+  hipLaunchKernelGGL(k_timestep<T>, dim3(nb), dim3(kRedBlock), 0, s, pos, hydro, hydro2, bdots, n, T(cn_vel),
+                     T(cn_eth), partials);
+  CS_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_final_min, dim3(1), dim3(kRedBlock), 0, s, partials, nb, round_float, dt_out);
+  CS_CHECK_LAUNCH();
+}
+template void launch_timestep<float>(const F4*, const F4*, const F4*, const F4*, int, float, float, int, double*,
+                                     double*, hipStream_t);
+template void launch_timestep<double>(const D4*, const D4*, const D4*, const D4*, int, float, float, int, double*,
+                                      double*, hipStream_t);
+
+void launch_timestep_mixed(const F4* pos, const double* u64, const F4* hydro2, const F4* bdots, int n,
+                           float cn_vel, float cn_eth, double* partials, double* dt_out, hipStream_t s) {
+  const int nb = ceil_div(n, kRedBlock);
+  hipLaunchKernelGGL(k_timestep_mixed, dim3(nb), dim3(kRedBlock), 0, s, pos, u64, hydro2, bdots, n, cn_vel, cn_eth,
+                     partials);
+  CS_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_final_min, dim3(1), dim3(kRedBlock), 0, s, partials, nb, 0, dt_out);
+  CS_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_gather(const R* __restrict__ src, R* __restrict__ dst,
+                                                    const KV* __restrict__ perm, int n) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t < n) dst[t] = src[perm[t].value];
+}
+
+template <class R>
+void launch_gather(const R* src, R* dst, const KV* perm, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather<R>, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, src, dst, perm, n);
+  CS_CHECK_LAUNCH();
+}
+template void launch_gather<F4>(const F4*, F4*, const KV*, int, hipStream_t);
+template void launch_gather<D4>(const D4*, D4*, const KV*, int, hipStream_t);
+template void launch_gather<double>(const double*, double*, const KV*, int, hipStream_t);
+template void launch_gather<uint32_t>(const uint32_t*, uint32_t*, const KV*, int, hipStream_t);
+
+__global__ __launch_bounds__(kBlock) void k_iota_kv(KV* kv, int n) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t < n) kv[t].value = (uint32_t)t;
+}
+void launch_iota_kv(KV* kv, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_iota_kv, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, kv, n);
+  CS_CHECK_LAUNCH();
+}
+
+}  // namespace cudasph

@@ -1,0 +1,120 @@
+// Host-side declarations of the kernel launchers (implemented in the .hip
+// files).  Every launcher enqueues on the given stream and checks the launch.
+#pragma once
+
+#include "common.h"
+
+namespace cudasph {
+
+struct WalkParams {
+  int n_nodes;        // real particles (the reference passes n_bodies, CUDA_calls.cpp:5)
+  int fix_phantom;
+  float rel_err;
+  float desired_ngb;
+  int max_iter;
+};
+
+struct EosConst;  // sph_math.h
+
+// ---- sph_passes.hip ----
+template <class T>
+void launch_solve_range(V4<T>* pos, int* n_ngb, const KV* hash, const V4<T>* tree, int n_dev,
+                        const WalkParams& wp, hipStream_t s);
+template <class T>
+void launch_find_ngb(const V4<T>* pos, const V4<T>* tree, KV* hash, const int* offset, const int* cap_count,
+                     int* count_out, int* list, int* flags, int n_dev, const WalkParams& wp, int clamp,
+                     hipStream_t s);
+template <class T, bool MIXED>
+void launch_density(const V4<T>* pos, const V4<T>* vel, const F4* pos_lo, V4<T>* hydro, V4<T>* dots,
+                    const KV* hash, const int* offset, const int* count, const int* list, int n_dev,
+                    hipStream_t s);
+template <class T>
+void launch_eos(V4<T>* hydro, int n_dev, const EosConst& c, hipStream_t s);
+void launch_eos_mixed(F4* hydro, const double* u64, int n, const EosConst& c, hipStream_t s);
+template <class T, bool MIXED>
+void launch_hydro(const V4<T>* pos, const V4<T>* vel, const F4* pos_lo, const V4<T>* hydro, const V4<T>* dots,
+                  V4<T>* bdots, V4<T>* hydro2, T* ngb_acc, const KV* hash, const int* offset, const int* count,
+                  const int* list, int n_dev, T alpha, T beta, hipStream_t s);
+
+// ---- gravity.hip ----
+template <class T>
+void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, int n_j, int soft, T eps,
+                    int exact, hipStream_t s);
+void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, int n_i, int n_j, int soft,
+                          float eps, hipStream_t s);
+
+// ---- sort.hip ----
+// exclusive scan of in[0..n) into out[0..n]; out[n] = total.  mode 1 scans
+// n_norm(in[i], 4) (the neighbour-list slot layout).  scratch holds
+// scan_scratch_ints(n) ints.
+long scan_scratch_ints(long n);
+void launch_exclusive_scan(const int* in, int* out, long n, int mode, int* scratch, hipStream_t s);
+// stable LSD radix sort of KV records by key bits [0, bits); the result is
+// left in `data` (alt is scratch of the same length).  scratch holds
+// radix_scratch_ints(n) ints.
+long radix_scratch_ints(long n);
+void launch_radix_sort(KV* data, KV* alt, long n, int bits, int* scratch, hipStream_t s);
+// list slots -> (key = neighbour j, value = slot k) for every slot of the
+// n particles' ranges (unused slot tails get key n); owner[k] = i
+void launch_reaction_pairs(const int* offset, const int* count, const int* list, KV* pairs, int* owner, int n,
+                           hipStream_t s);
+// per-j [start,end) of the sorted pairs (keys >= n skipped)
+void launch_reaction_bounds(const KV* pairs, long n_pairs, int n, int* start, int* end, hipStream_t s);
+// kv[t].key = keys[t]
+void launch_set_keys(KV* kv, const uint32_t* keys, int n, hipStream_t s);
+// bdots[j].xyz += ngb_acc[k] (r_i - r_j) over the sorted pairs of j, in
+// ascending k (sph_accelerations.cpp:104-115)
+template <class T>
+void launch_reaction(const V4<T>* pos, V4<T>* bdots, const T* ngb_acc, const KV* pairs, const int* owner,
+                     const int* start, const int* end, int n, hipStream_t s);
+void launch_reaction_mixed(const F4* pos, const F4* pos_lo, F4* bdots, const float* ngb_acc, const KV* pairs,
+                           const int* owner, const int* start, const int* end, int n, hipStream_t s);
+// Peano-Hilbert keys on the bounding cube (sort_bodies.cpp:121-163); keys
+// are 3*bits wide, written as (hi,lo) 32-bit halves
+template <class T>
+void launch_ph_keys(const V4<T>* pos, int n, int bits, uint32_t* key_hi, uint32_t* key_lo, int* scratch,
+                    hipStream_t s);
+
+// ---- integrate.hip ----
+template <class T>
+void launch_gravity_pack(const V4<T>* pos, const V4<T>* vel, V4<T>* gpos, V4<T>* gvel, int n, int n_dev,
+                         hipStream_t s);
+void launch_gravity_pack_mixed(const D4* pos64, const F4* pos, const F4* vel, F4* hi, F4* lo, F4* gvel, int n,
+                               int n_dev, hipStream_t s);
+template <class T>
+void launch_predict(V4<T>* pos, V4<T>* vel, V4<T>* hydro, V4<T>* bdots, V4<T>* grav, V4<T>* old_vu,
+                    const double* dt, int n, hipStream_t s);
+template <class T>
+void launch_correct(V4<T>* vel, V4<T>* hydro, const V4<T>* bdots, const V4<T>* old_vu, const double* dt_new,
+                    const double* dt_old, int n, hipStream_t s);
+void launch_predict_mixed(D4* pos64, D4* vel64, double* u64, D4* old64, F4* pos, F4* pos_lo, F4* vel, F4* hydro,
+                          F4* bdots, F4* grav, const double* dt, int n, hipStream_t s);
+void launch_correct_mixed(D4* vel64, double* u64, const D4* old64, F4* vel, F4* hydro, const F4* bdots,
+                          const double* dt_new, const double* dt_old, int n, hipStream_t s);
+void launch_refresh_mixed(const D4* pos64, const D4* vel64, const double* u64, F4* pos, F4* pos_lo, F4* vel,
+                          F4* hydro, int n, hipStream_t s);
+// dots += grav, and per-block partial sums of force and torque
+template <class T>
+void launch_add_gravity(V4<T>* bdots, const V4<T>* grav, const V4<T>* pos, const V4<T>* vel, int n,
+                        double* partials, double* out6, hipStream_t s);
+// dt = min_i 1/(1/dt_vel + 1/dt_eth) (compute_timestep.cpp:3-25); round_float
+// stores the float-rounded value as the reference returns float
+template <class T>
+void launch_timestep(const V4<T>* pos, const V4<T>* hydro, const V4<T>* hydro2, const V4<T>* bdots, int n,
+                     float cn_vel, float cn_eth, int round_float, double* partials, double* dt_out,
+                     hipStream_t s);
+void launch_timestep_mixed(const F4* pos, const double* u64, const F4* hydro2, const F4* bdots, int n,
+                           float cn_vel, float cn_eth, double* partials, double* dt_out, hipStream_t s);
+// T U W Jx Jy Jz sums
+template <class T>
+void launch_stats(const V4<T>* pos, const V4<T>* vel, const V4<T>* hydro, const V4<T>* grav, int n,
+                  double* partials, double* out6, hipStream_t s);
+void launch_stats_mixed(const D4* pos64, const D4* vel64, const double* u64, const F4* grav, int n,
+                        double* partials, double* out6, hipStream_t s);
+long reduce_partials(int n);   // doubles needed for partial sums of n items (x6 components)
+// permutation gather of a 4-record array: dst[t] = src[perm[t]]
+template <class R>
+void launch_gather(const R* src, R* dst, const KV* perm, int n, hipStream_t s);
+void launch_iota_kv(KV* kv, int n, hipStream_t s);
+
+}  // namespace cudasph

@@ -1,0 +1,425 @@
+// SPH passes on the GPU: kd-tree walk, smoothing-length solve, neighbour
+// lists, density, EOS and hydro accelerations.
+//
+// One lane per particle.  Lanes take particles in the order of the hash
+// array (thread t -> bodies_hash[t].value, as in the reference); the engine
+// fills it with kd-tree heap order, so the 64 lanes of a wavefront hold
+// spatially adjacent particles and walk nearly the same nodes, and their
+// neighbour gathers hit the same cache lines.  The per-lane walk stack lives
+// in LDS, one column per lane (bank-conflict free).  Per-particle sums run in
+// list order, so each particle's result is the same bit pattern as the CPU
+// restatement's.
+#include "kernels.h"
+#include "sph_math.h"
+
+namespace cudasph {
+
+// ---------------------------------------------------------------------------
+// kd-tree walk (dev_solve_range.cu:17-75, dev_find_ngb.cu:13-73).
+// Pop a node; push its left child if the query sphere reaches below the split
+// plane and its right child if it reaches above (so the right subtree is
+// visited first); then test the node's own particle.  Children are pushed for
+// every node with 2p <= n_nodes, which for even n_nodes includes the
+// nonexistent node n+1; the tree buffer holds a zero record there, which is
+// what the reference's out-of-range texture fetch returns.  fix_phantom drops
+// that push instead.  Returns the number of nodes visited.
+template <class T, class Visit>
+__device__ __forceinline__ int kd_walk(const V4<T>& q, const V4<T>* __restrict__ tree, int n_nodes,
+                                       int fix_phantom, int* __restrict__ stk, Visit&& visit) {
+  int top = 0;
+  stk[0] = 1;
+  int n_walk = 0;
+  const T r2 = q.w * q.w;
+  while (top >= 0) {
+    ++n_walk;
+    int id = stk[top * kBlock];
+    const V4<T> node = tree[id];
+    const int bits = node_bits(node.w);
+    const int split = (bits >> 24) & 3;
+    id <<= 1;
+    if (id <= n_nodes && split < 3) {
+      const T qc = split == 0 ? q.x : (split == 1 ? q.y : q.z);
+      const T nc = split == 0 ? node.x : (split == 1 ? node.y : node.z);
+      if (qc - q.w < nc) stk[(top++) * kBlock] = id;
+      if (qc + q.w > nc && (!fix_phantom || id < n_nodes)) stk[(top++) * kBlock] = id + 1;
+    }
+    --top;
+    const T dx = q.x - node.x, dy = q.y - node.y, dz = q.z - node.z;
+    const T ds2 = dx * dx + dy * dy + dz * dz;
+    if (ds2 < r2) visit(ds2, bits & ~(3 << 24));
+  }
+  return n_walk;
+}
+
+template <class T>
+struct NgbEval {
+  T f, step, count;
+};
+
+// smoothed count minus target, Newton step and integer count at range q.w
+// (dev_solve_range.cu:17-89)
+template <class T>
+__device__ __forceinline__ NgbEval<T> eval_ngb(const V4<T>& q, const V4<T>* __restrict__ tree,
+                                               const WalkParams& wp, int* stk) {
+  T f = T(0), df = T(0), cnt = T(0);
+  kd_walk<T>(q, tree, wp.n_nodes, wp.fix_phantom, stk, [&](T ds2, int) {
+    const T u = T(2) * dsqrt(ds2) / q.w;
+    f += kern_ig<T>(u);
+    df -= kern_digdh<T>(u);
+    cnt += T(1);
+  });
+  f -= T(wp.desired_ngb);
+  df = T((double)df * (1.0 / (double)q.w));
+  if (df != T(0)) df = f / df;
+  else df = f < T(0) ? -q.w : q.w;
+  return {f, df, cnt};
+}
+
+template <class T>
+__device__ __forceinline__ T clamp_step(T dx, T range) {   // f_scale = 0.1, dev_solve_range.cu:92,100-103
+  if ((double)dabs(dx) > 0.1 * (double)range) dx = T((dx < T(0) ? -0.1 : 0.1) * (double)range);
+  return dx;
+}
+
+// bracket [lo, hi] around the target (dev_solve_range.cu:93-135); z = count
+template <class T>
+__device__ __forceinline__ V4<T> bracket(V4<T> q, const V4<T>* __restrict__ tree, const WalkParams& wp,
+                                         int* stk) {
+  const T rel = T(wp.rel_err);
+  NgbEval<T> f = eval_ngb<T>(q, tree, wp, stk);
+  T dx = f.step;
+  if (dabs(dx) < rel * q.w)
+    return {T((double)q.w - 0.5 * (double)dabs(dx)), T((double)q.w + 0.5 * (double)dabs(dx)), f.count, T(0)};
+  dx = clamp_step<T>(dx, q.w);
+  T x_prev = q.w, f_prev = f.f;
+  int iter = 0;
+  while ((f_prev * f.f >= T(0)) && (iter++ < wp.max_iter)) {
+    x_prev = q.w;
+    f_prev = f.f;
+    q.w -= dx;
+    f = eval_ngb<T>(q, tree, wp, stk);
+    dx = f.step;
+    if (dabs(dx) < rel * q.w)
+      return {T((double)q.w - 0.5 * (double)dabs(dx)), T((double)q.w + 0.5 * (double)dabs(dx)), f.count, T(0)};
+    dx = clamp_step<T>(dx, q.w);
+  }
+  if (x_prev < q.w) return {x_prev, q.w, f.count, T(0)};
+  return {q.w, x_prev, f.count, T(0)};
+}
+
+// safeguarded Newton inside the bracket (dev_solve_range.cu:137-172)
+template <class T>
+__device__ __forceinline__ void solve_range(V4<T> q, const V4<T>* __restrict__ tree, const WalkParams& wp,
+                                            int* stk, T& range, T& count) {
+  const T rel = T(wp.rel_err);
+  V4<T> br = bracket<T>(q, tree, wp, stk);
+  T lo = br.x, hi = br.y;
+  count = br.z;
+  q.w = T(0.5 * (double)(lo + hi));
+  T dx = T(0);
+  if (dabs(lo - hi) > rel * q.w) {
+    bool keep = true;
+    int iter = 0;
+    while (keep && (iter++ < wp.max_iter)) {
+      q.w -= dx;
+      if ((q.w - hi) * (q.w - lo) >= T(0)) {          // left the bracket: bisect
+        dx = T(0.5 * (double)(hi - lo));
+        q.w = T(0.5 * (double)(lo + hi));
+      }
+      const NgbEval<T> f = eval_ngb<T>(q, tree, wp, stk);
+      count = f.count;
+      if (f.f < T(0)) lo = q.w;
+      else hi = q.w;
+      dx = f.step;
+      keep = dabs(dx) > rel * q.w;
+    }
+  }
+  range = q.w;
+}
+
+// dev_solve_range.cu:174-189
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_solve_range(V4<T>* __restrict__ pos, int* __restrict__ n_ngb,
+                                                         const KV* __restrict__ hash,
+                                                         const V4<T>* __restrict__ tree, int n_dev,
+                                                         WalkParams wp) {
+  __shared__ int s_stack[kStackDepth * kBlock];
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n_dev) return;
+  const int i = (int)hash[t].value;
+  const V4<T> q = pos[i];
+  T range = T(0), count = T(0);
+  if (q.w > T(0)) solve_range<T>(q, tree, wp, s_stack + threadIdx.x, range, count);
+  pos[i].w = range;
+  n_ngb[i] = (int)count;
+}
+
+// dev_find_ngb.cu:75-87.  Capacity of particle i's slot is
+// n_norm(cap_count[i], 4) (the layout of sph_accelerations.cpp:58-64); at most
+// that many ids are written, the true count goes to count_out[i], and
+// flags[0] is raised when a count exceeds its slot.  With clamp set,
+// count_out is clamped to the slot (for callers that own the offsets).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_find_ngb(const V4<T>* __restrict__ pos, const V4<T>* __restrict__ tree,
+                                                      KV* __restrict__ hash, const int* __restrict__ offset,
+                                                      const int* cap_count, int* count_out,
+                                                      int* __restrict__ list, int* __restrict__ flags,
+                                                      int n_dev, WalkParams wp, int clamp) {
+  __shared__ int s_stack[kStackDepth * kBlock];
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n_dev) return;
+  const int i = (int)hash[t].value;
+  const V4<T> q = pos[i];
+  const int off = offset[i];
+  const int cap = n_norm(cap_count[i], 4);
+  int c = 0;
+  const int nw = kd_walk<T>(q, tree, wp.n_nodes, wp.fix_phantom, s_stack + threadIdx.x, [&](T, int id) {
+    if (c < cap) list[off + c] = id;
+    ++c;
+  });
+  hash[t].key = (uint32_t)nw;
+  if (c > cap) {
+    atomicOr(flags, 1);
+    if (clamp) c = cap;
+  }
+  count_out[i] = c;
+}
+
+// ---------------------------------------------------------------------------
+// compensated (two-sum) accumulator for the mixed form
+struct KSum {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ float get() const { return s; }
+};
+template <class T, bool MIXED>
+struct Sum;
+template <class T>
+struct Sum<T, false> {
+  T s = T(0);
+  __device__ __forceinline__ void add(T x) { s += x; }
+  __device__ __forceinline__ void sub(T x) { s -= x; }
+  __device__ __forceinline__ T get() const { return s; }
+};
+template <>
+struct Sum<float, true> {
+  KSum k;
+  __device__ __forceinline__ void add(float x) { k.add(x); }
+  __device__ __forceinline__ void sub(float x) { k.add(-x); }
+  __device__ __forceinline__ float get() const { return k.get(); }
+};
+
+// position difference r_i - r_j; the mixed form adds the low words of the
+// split coordinates
+template <class T, bool MIXED>
+__device__ __forceinline__ void pos_diff(const V4<T>& pi, const V4<T>& pj, const F4* lo, int i, int j,
+                                         const F4& loi, T& dx, T& dy, T& dz) {
+  dx = pi.x - pj.x;
+  dy = pi.y - pj.y;
+  dz = pi.z - pj.z;
+  if constexpr (MIXED) {
+    const F4 lj = lo[j];
+    dx += loi.x - lj.x;
+    dy += loi.y - lj.y;
+    dz += loi.z - lj.z;
+  }
+}
+
+// density, grad-h terms, div v and |curl v| (dev_density_loop.cu:29-111)
+template <class T, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_density(const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel,
+                                                     const F4* __restrict__ pos_lo, V4<T>* __restrict__ hydro,
+                                                     V4<T>* __restrict__ dots, const KV* __restrict__ hash,
+                                                     const int* __restrict__ offset, const int* __restrict__ count,
+                                                     const int* __restrict__ list, int n_dev) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n_dev) return;
+  const int i = (int)hash[t].value;
+  const V4<T> bp = pos[i], bv = vel[i];
+  F4 loi{};
+  if constexpr (MIXED) loi = pos_lo[i];
+  Sum<T, MIXED> rho, om, psi, den, divv, cx, cy, cz;
+  const int off = offset[i], n = count[i];
+  for (int k = off; k < off + n; ++k) {
+    const int j = list[k];
+    const V4<T> pj = pos[j], vj = vel[j];
+    T dx, dy, dz;
+    pos_diff<T, MIXED>(bp, pj, pos_lo, i, j, loi, dx, dy, dz);
+    const T u = T(2) * dsqrt(dx * dx + dy * dy + dz * dz) / bp.w;
+    rho.add(vj.w * kern_w<T>(u));
+    om.add(vj.w * kern_dwdh<T>(u));
+    psi.add(vj.w * kern_dphidh<T>(u));
+    den.sub(kern_digdh<T>(u));
+    const T dvx = bv.x - vj.x, dvy = bv.y - vj.y, dvz = bv.z - vj.z;
+    const T mdw = bv.w * kern_dw<T>(u);
+    divv.add(mdw * (dx * dvx + dy * dvy + dz * dvz));
+    cx.add(mdw * (dy * dvz - dz * dvy));
+    cy.add(mdw * (dz * dvx - dx * dvz));
+    cz.add(mdw * (dx * dvy - dy * dvx));
+  }
+  const T h = T(0.5 * (double)bp.w);
+  const T h3 = h * h * h, h5 = h * h * h * h * h;
+  T o = om.get(), p = psi.get();
+  const T d = den.get();
+  if (d != T(0)) {
+    o = T((double)o * (1.0 / (double)h3 / (double)d));
+    p = T((double)p * (1.0 / (double)h / (double)d));
+  } else {
+    o = p = T(0);
+  }
+  const T r = T((double)rho.get() * (1.0 / (double)h3));
+  const T dv = T((double)divv.get() * (1.0 / (double)r / (double)h5));
+  const T x = cx.get(), y = cy.get(), z = cz.get();
+  const T curl = dsqrt(x * x + y * y + z * z) / r / h5;
+  hydro[i].x = r;
+  dots[i] = {dv, curl, o, p};
+}
+
+// dev_compute_pressure.cu:99-111 (every record, no hash, as the reference)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_eos(V4<T>* __restrict__ hydro, int n_dev, EosConst c) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n_dev) return;
+  hydro[i].y = eos_record<T>(hydro[i], c);
+}
+
+// mixed form: P from the FP64 internal energy with the FP64 Newton root
+__global__ __launch_bounds__(kBlock) void k_eos_mixed(F4* __restrict__ hydro, const double* __restrict__ u64,
+                                                       int n, EosConst c) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const F4 hd = hydro[i];
+  hydro[i].y = (float)eos_pressure_fp64((double)hd.x, u64[i], (double)hd.w, c);
+}
+
+// hydro accelerations, du/dt and max signal speed (dev_hydro_dots.cu:25-113);
+// writes the per-pair reaction coefficient m_i*acc to ngb_acc[k]
+template <class T, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_hydro(const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel,
+                                                   const F4* __restrict__ pos_lo, const V4<T>* __restrict__ hydro,
+                                                   const V4<T>* __restrict__ dots, V4<T>* __restrict__ bdots,
+                                                   V4<T>* __restrict__ hydro2, T* __restrict__ ngb_acc,
+                                                   const KV* __restrict__ hash, const int* __restrict__ offset,
+                                                   const int* __restrict__ count, const int* __restrict__ list,
+                                                   int n_dev, T alpha, T beta) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n_dev) return;
+  const int i = (int)hash[t].value;
+  const V4<T> bp = pos[i], bv = vel[i], hd = hydro[i], dd = dots[i];
+  F4 loi{};
+  if constexpr (MIXED) loi = pos_lo[i];
+  const T prho2 = hd.y / (hd.x * hd.x);
+  const double hw = 0.5 * (double)bp.w;
+  // POW5(0.5*range) and POW2(0.5*range) expand left to right in the reference
+  const double pow5 = hw * 0.5 * (double)bp.w * 0.5 * (double)bp.w * 0.5 * (double)bp.w * 0.5 * (double)bp.w;
+  const double pow2 = hw * 0.5 * (double)bp.w;
+  Sum<T, MIXED> ax, ay, az, ud;
+  T vsig = T(0);
+  const int off = offset[i], n = count[i];
+  for (int k = off; k < off + n; ++k) {
+    const int j = list[k];
+    const V4<T> pj = pos[j], vj = vel[j], hj = hydro[j], dj = dots[j];
+    T dx, dy, dz;
+    pos_diff<T, MIXED>(bp, pj, pos_lo, i, j, loi, dx, dy, dz);
+    const T dvx = bv.x - vj.x, dvy = bv.y - vj.y, dvz = bv.z - vj.z;
+    const T ds = dsqrt(dx * dx + dy * dy + dz * dz);
+    const T u = T((double)ds / hw);
+    const T drdv = dx * dvx + dy * dvy + dz * dvz;
+    const Visc<T> av = art_visc<T>(bp.w, pj.w, ds, drdv, hd, dd, hj, dj, alpha, beta);
+    vsig = dmax(vsig, av.vsig);
+    const double w1 = (double)kern_dw<T>(u);
+    const T g = kern_dig<T>(u);
+    const double pv = (double)prho2 + 0.5 * (double)av.pi;
+    T acc = T(pv * w1 / pow5);
+    acc = T((double)acc - ((double)(prho2 * dd.z) + 0.5 * (double)dd.w) / (double)vj.w * (double)g / pow2);
+    ax.sub(vj.w * acc * dx);
+    ay.sub(vj.w * acc * dy);
+    az.sub(vj.w * acc * dz);
+    ngb_acc[k] = bv.w * acc;
+    T udot = T((double)vj.w * pv * w1 / pow5);
+    udot = T((double)udot - (double)(prho2 * dd.z * g) / pow2);
+    ud.add(udot * drdv);
+  }
+  bdots[i] = {ax.get(), ay.get(), az.get(), ud.get()};
+  hydro2[i].x = vsig;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+template <class T>
+void launch_solve_range(V4<T>* pos, int* n_ngb, const KV* hash, const V4<T>* tree, int n_dev,
+                        const WalkParams& wp, hipStream_t s) {
+  hipLaunchKernelGGL(k_solve_range<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, n_ngb, hash,
+                     tree, n_dev, wp);
+  CS_CHECK_LAUNCH();
+}
+
+template <class T>
+void launch_find_ngb(const V4<T>* pos, const V4<T>* tree, KV* hash, const int* offset, const int* cap_count,
+                     int* count_out, int* list, int* flags, int n_dev, const WalkParams& wp, int clamp,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_find_ngb<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, tree, hash, offset,
+                     cap_count, count_out, list, flags, n_dev, wp, clamp);
+  CS_CHECK_LAUNCH();
+}
+
+template <class T, bool MIXED>
+void launch_density(const V4<T>* pos, const V4<T>* vel, const F4* pos_lo, V4<T>* hydro, V4<T>* dots,
+                    const KV* hash, const int* offset, const int* count, const int* list, int n_dev,
+                    hipStream_t s) {
+  hipLaunchKernelGGL((k_density<T, MIXED>), dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, vel,
+                     pos_lo, hydro, dots, hash, offset, count, list, n_dev);
+  CS_CHECK_LAUNCH();
+}
+
+template <class T>
+void launch_eos(V4<T>* hydro, int n_dev, const EosConst& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_eos<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, hydro, n_dev, c);
+  CS_CHECK_LAUNCH();
+}
+
+void launch_eos_mixed(F4* hydro, const double* u64, int n, const EosConst& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_eos_mixed, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, hydro, u64, n, c);
+  CS_CHECK_LAUNCH();
+}
+
+template <class T, bool MIXED>
+void launch_hydro(const V4<T>* pos, const V4<T>* vel, const F4* pos_lo, const V4<T>* hydro, const V4<T>* dots,
+                  V4<T>* bdots, V4<T>* hydro2, T* ngb_acc, const KV* hash, const int* offset, const int* count,
+                  const int* list, int n_dev, T alpha, T beta, hipStream_t s) {
+  hipLaunchKernelGGL((k_hydro<T, MIXED>), dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, vel, pos_lo,
+                     hydro, dots, bdots, hydro2, ngb_acc, hash, offset, count, list, n_dev, alpha, beta);
+  CS_CHECK_LAUNCH();
+}
+
+#define CS_INST(T)                                                                                        \
+  template void launch_solve_range<T>(V4<T>*, int*, const KV*, const V4<T>*, int, const WalkParams&,      \
+                                      hipStream_t);                                                       \
+  template void launch_find_ngb<T>(const V4<T>*, const V4<T>*, KV*, const int*, const int*, int*, int*,   \
+                                   int*, int, const WalkParams&, int, hipStream_t);                       \
+  template void launch_eos<T>(V4<T>*, int, const EosConst&, hipStream_t);
+CS_INST(float)
+CS_INST(double)
+#undef CS_INST
+template void launch_density<float, false>(const F4*, const F4*, const F4*, F4*, F4*, const KV*, const int*,
+                                           const int*, const int*, int, hipStream_t);
+template void launch_density<float, true>(const F4*, const F4*, const F4*, F4*, F4*, const KV*, const int*,
+                                          const int*, const int*, int, hipStream_t);
+template void launch_density<double, false>(const D4*, const D4*, const F4*, D4*, D4*, const KV*, const int*,
+                                            const int*, const int*, int, hipStream_t);
+template void launch_hydro<float, false>(const F4*, const F4*, const F4*, const F4*, const F4*, F4*, F4*,
+                                         float*, const KV*, const int*, const int*, const int*, int, float,
+                                         float, hipStream_t);
+template void launch_hydro<float, true>(const F4*, const F4*, const F4*, const F4*, const F4*, F4*, F4*,
+                                        float*, const KV*, const int*, const int*, const int*, int, float,
+                                        float, hipStream_t);
+template void launch_hydro<double, false>(const D4*, const D4*, const F4*, const D4*, const D4*, D4*, D4*,
+                                          double*, const KV*, const int*, const int*, const int*, int, double,
+                                          double, hipStream_t);
+
+}  // namespace cudasph

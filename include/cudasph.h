@@ -1,0 +1,180 @@
+/*
+ * cudasph.h -- C ABI of the MI355X-native SPH engine (libcudasph.so).
+ *
+ * Two groups of entry points:
+ *
+ *  1. The reference's own boundary, with the same names, argument order and
+ *     meaning, so that the reference's host code (CUDAsph class) links
+ *     against this library unchanged.  Each declaration cites the reference
+ *     line it replaces.  float4 arguments are 16-byte {x,y,z,w} float records
+ *     and KeyValuePair is {uint32 key, uint32 value}, 8-byte aligned; both are
+ *     declared here as plain structs so this header needs no GPU headers.
+ *     Errors behave as in the reference: a failing HIP call prints the error
+ *     and exits the process (host_calls.cu:14-22).
+ *
+ *  2. The engine API (sph_engine_*): a device-resident simulation in one of
+ *     three forms (plain FP32 drop-in, mixed precision, FP64), its passes,
+ *     state transfer and diagnostics, plus the adjoint.  These return 0 / a
+ *     byte count on success and a negative value on failure; the message is
+ *     available from sph_last_error().  They never exit the process.
+ *
+ * All pointers in group 1 are device pointers unless the name says host.
+ */
+#ifndef CUDASPH_H
+#define CUDASPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sph_float4 { float x, y, z, w; } sph_float4;
+typedef struct sph_kv { uint32_t key, value; } sph_kv; /* KeyValuePair, radixsort.cuh:48-57 */
+
+/* ---------------------------------------------------------------------- */
+/* 1. reference boundary (CUDAsph.h:22-48, radixsort.cuh:59-61)            */
+
+/* CUDAsph.h:24, host_calls.cu:30-32 -- no-op */
+void initCUDA(void);
+/* CUDAsph.h:25, host_calls.cu:55-59 -- device allocation of n bytes */
+void allocateCUDAarray(void** pos, int n, int device_id);
+/* CUDAsph.h:26, host_calls.cu:60-64 */
+void deleteCUDAarray(void* pos, int device_id);
+/* CUDAsph.h:27, host_calls.cu:65-69 -- n bytes host -> device */
+void copyArrayToDevice(void* device, const void* host, int n, int device_id);
+/* CUDAsph.h:28, host_calls.cu:70-74 -- n bytes device -> host */
+void copyArrayFromDevice(void* host, const void* device, int n, int device_id);
+/* CUDAsph.h:30-31, host_calls.cu:319-361 -- softened direct-N gravity.
+ * pos = n x {x,y,z,m}, vel = n x {vx,vy,vz,range^2}; accel = n x {gx,gy,gz,phi}.
+ * p and q are the reference's launch shape; this library chooses its own
+ * (p must still divide n, as in the reference). */
+void compute_CUDA_gravity(int numBodies, sph_float4* accel, float* pos, float* vel,
+                          int p, int q, int device_id);
+/* CUDAsph.h:35-39, host_calls.cu:292-317 -- 16-byte-per-body arrays */
+void zallocateNBodyArrays(float** pos, int numBodies, int device_id);
+void zdeleteNBodyArrays(float* pos, int device_id);
+void zcopyArrayFromDevice(float* host, const float* device, int numBodies, int device_id);
+void zcopyArrayToDevice(float* device, const float* host, int numBodies, int device_id);
+/* CUDAsph.h:41, host_calls.cu:77-153 -- smoothing-length solve.  In place:
+ * bodies_pos[i].w <- range (= 2h), n_ngb[i] <- neighbour count at that range.
+ * Binds pos/vel/n_ngb/kd_tree for the following host_sph_accelerations call,
+ * as the reference's texture bindings do.  kd_tree has n_bodies+1 records
+ * (index 0 unused).  Returns elapsed wall seconds. */
+float host_solve_range(int n_bodies, int n_nodes, float rel_err, sph_kv* bodies_hash,
+                       int* bodies_map, sph_float4* bodies_pos, sph_float4* bodies_vel,
+                       sph_float4* kd_tree, int* n_ngb, int device_id);
+/* CUDAsph.h:42-43, host_calls.cu:155-287 -- neighbour lists, density, EOS and
+ * hydro accelerations over the arrays bound by host_solve_range.  On return
+ * n_ngb holds the number of ids find_ngb wrote for each particle (the
+ * h-solve count except in the reference's bracket edge case), and
+ * bodies_hash[t].key holds the walk length of thread t.  Returns wall seconds. */
+float host_sph_accelerations(int n_bodies, int ngb_tot, float av_alpha, float av_beta,
+                             sph_kv* bodies_hash, sph_kv* bodies_hash_extra, int* bodies_map,
+                             int* ngb_list, int* ngb_offset, sph_float4* hydro_data,
+                             sph_float4* dots_data, sph_float4* bodies_dots,
+                             sph_float4* hydro_data2, float* ngb_acc, int device_id);
+/* CUDAsph.h:44, host_calls.cu:364-373 */
+int get_device_count(void);
+/* CUDAsph.h:45, host_calls.cu:47-53 -- no-op, as in the reference */
+void cuda_set_device(int device_id);
+/* CUDAsph.h:46, host_calls.cu:39-45 -- 1-based device selection */
+void cuda_set_device0(int device_id);
+/* CUDAsph.h:47, host_calls.cu:33-38 */
+int cuda_get_device(void);
+/* radixsort.cuh:59-61, radixsort.cu:50-77 -- stable LSD sort of
+ * (key,value) pairs by the low `bits` bits of key; pData1 is scratch of the
+ * same size.  Sorted output ends in pData0. */
+void RadixSort(sph_kv* pData0, sph_kv* pData1, unsigned int elements, unsigned int bits);
+
+/* No counterpart in the reference (its DESIRED_NGB, MAX_ITER, EOS unit
+ * constants and M4 softening are compile-time constants, dev_solve_range.cu:14-15,
+ * dev_compute_pressure.cu:55-81): sets the values the group-1 entry points use.
+ * exact_rsqrt = 1 makes compute_CUDA_gravity use a correctly rounded 1/sqrt. */
+struct sph_params;
+void sph_abi_set_params(const struct sph_params* p, int exact_rsqrt);
+
+/* ---------------------------------------------------------------------- */
+/* 2. engine API                                                           */
+
+/* Parameters.  Defaults (sph_params_default) are the reference's values;
+ * sph_params_preset gives the five benchmark configurations. */
+typedef struct sph_params {
+  int    n_bodies;       /* real particles N                                  */
+  int    n_dev;          /* padded count, n_norm(N,256) (read_state.cpp:27)    */
+  float  rel_err;        /* h-solve tolerance, 1e-3 (CUDA_calls.cpp:4)         */
+  float  desired_ngb;    /* smoothed neighbour target, 24 (dev_solve_range.cu:14) */
+  int    max_iter;       /* 20 (dev_solve_range.cu:15)                         */
+  int    fix_phantom;    /* 0: the walk reads the zero node n+1 like the reference */
+  float  av_alpha;       /* 1 (CUDA_calls.cpp:13)                              */
+  float  av_beta;        /* 2 (CUDA_calls.cpp:14)                              */
+  float  cn_vel;         /* 0.3 (compute_timestep.cpp:4)                       */
+  float  cn_eth;         /* 0.2 (compute_timestep.cpp:5)                       */
+  int    grav_mode;      /* 0: M4 spline at range (reference), 1: Plummer eps  */
+  float  grav_eps;       /* Plummer softening length                           */
+  double eos_density_unit, eos_energy_unit, eos_pressure_unit; /* dev_compute_pressure.cu:55-81 */
+  double eos_amu, eos_boltzmann, eos_rad_const, eos_gamma, eos_temp_base;
+  int    ph_period;      /* Peano-Hilbert re-sort period, 100 (step.cpp:26)    */
+  int    ph_bits;        /* bits per dimension, 18 (sort_bodies.cpp:3)         */
+  int    eos_newton;     /* 0: Ferrari closed form (reference), 1: FP64 Newton */
+  int    n_threads;      /* host threads for the kd-tree build (0 = all)       */
+} sph_params;
+
+enum {
+  SPH_FORM_PLAIN = 0,    /* FP32 drop-in with the reference's arithmetic       */
+  SPH_FORM_MIXED = 1,    /* FP32 pair math, compensated sums, hi/lo positions,
+                            FP64 integration, EOS Newton and diagnostics      */
+  SPH_FORM_FP64 = 2      /* every pass in FP64                                 */
+};
+
+/* pass ids (same numbering as the oracle) */
+enum {
+  SPH_TREE = 0, SPH_SOLVE = 1, SPH_OFFSETS = 2, SPH_FIND = 3, SPH_DENSITY = 4,
+  SPH_EOS = 5, SPH_HYDRO = 6, SPH_REACTION = 7, SPH_GRAVITY = 8,
+  SPH_ADD_GRAVITY = 9, SPH_TIMESTEP = 10, SPH_PREDICT = 11, SPH_CORRECT = 12,
+  SPH_STATS = 13, SPH_PH_SORT = 14, SPH_SPH_ACC = 15, SPH_INIT = 16, SPH_STEP = 17
+};
+
+void        sph_params_default(sph_params* p, int n_bodies);
+/* config 1..5 of the benchmark set; n_bodies <= 0 takes the config's own N */
+int         sph_params_preset(sph_params* p, int config, int n_bodies);
+/* Seeded initial conditions for config 1..5 in FP64: pos4 = {x,y,z,range},
+ * vel4 = {vx,vy,vz,m}, hydro4 = {rho,P,u,mu}; n records each.  Identical on
+ * every x86-64 host with the same libm. */
+int         sph_ic_generate(int config, int n, uint64_t seed, double* pos4, double* vel4,
+                            double* hydro4);
+
+const char* sph_last_error(void);
+
+void*  sph_engine_create(int form, const sph_params* p, int device_id);
+void   sph_engine_destroy(void* e);
+/* State arrays by name, in the engine's real type (float for PLAIN, double
+ * for FP64, and for MIXED: float for the SPH arrays plus "pos64", "vel64",
+ * "u64" in double).  Names: pos vel hydro dots bdots hydro2 grav (4 reals per
+ * particle), tree (n_dev+2 records), hash, n_ngb, ngb_count, ngb_offset
+ * (n_dev+1), ngb_list, ngb_acc.  Returns bytes copied or < 0. */
+long   sph_engine_set(void* e, const char* name, const void* host, long n_bytes);
+long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
+double sph_engine_get_scalar(void* e, const char* name);
+int    sph_engine_set_scalar(void* e, const char* name, double v);
+int    sph_engine_pass(void* e, int pass);
+/* run `steps` full steps (step.cpp:13-147) */
+int    sph_engine_run(void* e, int steps);
+int    sph_engine_sync(void* e);
+/* T U W E |J| Fx Fy Fz Tx Ty Tz of the last STATS / ADD_GRAVITY passes */
+int    sph_engine_stats(void* e, double* out11);
+/* Per-kernel GPU milliseconds accumulated since the last reset, measured with
+ * HIP events on the engine's stream; names[i] is a static string. */
+int    sph_engine_timers(void* e, const char** names, double* ms, long* calls, int max_n);
+void   sph_engine_reset_timers(void* e);
+
+/* Reference state file (read_state.cpp:5-67 / write_state.cpp:6-61): 124-byte
+ * header + 23 doubles per particle. */
+int    sph_state_file_count(const char* path);   /* N from the header, or < 0 */
+int    sph_engine_read_state(void* e, const char* path);
+int    sph_engine_write_state(void* e, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
