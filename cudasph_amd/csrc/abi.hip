@@ -135,6 +135,10 @@ float host_solve_range(int n_bodies, int n_nodes, float rel_err, sph_kv* bodies_
                        sph_float4* bodies_pos, sph_float4* bodies_vel, sph_float4* kd_tree, int* n_ngb, int) {
   const double t0 = wall();
   AbiState& a = abi();
+  if (n_nodes < 0 || n_nodes > n_bodies || n_bodies >= (1 << 24) - 2) {
+    std::fprintf(stderr, "host_solve_range: need 0 <= n_nodes <= n_bodies < 2^24\n");
+    std::exit(1);
+  }
   guarded([&] {
     const size_t need = (size_t)n_bodies + 2;
     if (a.tree_cap < need) {

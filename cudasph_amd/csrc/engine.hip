@@ -143,6 +143,7 @@ class Engine final : public EngineBase {
   void tree();
   void solve();
   void offsets();
+  long check_total();
   void find();
   void density();
   void eos();
@@ -401,10 +402,27 @@ void Engine<T>::solve() {
   launch_solve_range<T>(pos_, n_ngb_, hash_, tree_, nd_, walk_params(), s_);
 }
 
+// list slot layout (sph_accelerations.cpp:58-72); the total is checked
+// against the allocation before any list is written
 template <class T>
 void Engine<T>::offsets() {
-  TimerScope ts(timers_, "offsets_scan", s_);
-  launch_exclusive_scan(n_ngb_, ngb_offset_, nd_, 1, scan_tmp_, s_);
+  {
+    TimerScope ts(timers_, "offsets_scan", s_);
+    launch_exclusive_scan(n_ngb_, ngb_offset_, nd_, 1, scan_tmp_, s_);
+  }
+  check_total();
+}
+
+template <class T>
+long Engine<T>::check_total() {
+  int tot = 0;
+  CS_HIP(hipMemcpyAsync(&h_small_[1], ngb_offset_.p + nd_, 4, hipMemcpyDeviceToHost, s_));
+  sync();
+  std::memcpy(&tot, &h_small_[1], 4);
+  if (tot < 0 || tot > cap_)
+    throw std::runtime_error("neighbour list overflow: ngb_tot exceeds 100 n_dev (sph_accelerations.cpp:66-72)");
+  ngb_tot_ = tot;
+  return tot;
 }
 
 // dev_find_ngb; if a particle finds more ids than its slot holds (the
@@ -413,6 +431,7 @@ void Engine<T>::offsets() {
 template <class T>
 void Engine<T>::find() {
   const WalkParams wp = walk_params();
+  check_total();
   for (int attempt = 0; attempt < 2; ++attempt) {
     {
       TimerScope ts(timers_, "find_ngb", s_);
@@ -420,21 +439,14 @@ void Engine<T>::find() {
       launch_find_ngb<T>(pos_, tree_, hash_, ngb_offset_, attempt == 0 ? n_ngb_.p : ngb_count_.p, ngb_count_,
                          list_, flags_, nd_, wp, 0, s_);
     }
-    int fl = 0, tot = 0;
+    int fl = 0;
     CS_HIP(hipMemcpyAsync(&h_small_[0], flags_, 4, hipMemcpyDeviceToHost, s_));
-    CS_HIP(hipMemcpyAsync(&h_small_[1], ngb_offset_.p + nd_, 4, hipMemcpyDeviceToHost, s_));
     sync();
     std::memcpy(&fl, &h_small_[0], 4);
-    std::memcpy(&tot, &h_small_[1], 4);
-    ngb_tot_ = tot;
-    if (ngb_tot_ > cap_) throw std::runtime_error("neighbour list overflow: ngb_tot > 100 n_dev (sph_accelerations.cpp:66-72)");
     if (!fl) return;
     ++overflow_reruns_;
     launch_exclusive_scan(ngb_count_, ngb_offset_, nd_, 1, scan_tmp_, s_);
-    CS_HIP(hipMemcpyAsync(&h_small_[1], ngb_offset_.p + nd_, 4, hipMemcpyDeviceToHost, s_));
-    sync();
-    std::memcpy(&tot, &h_small_[1], 4);
-    if (tot > cap_) throw std::runtime_error("neighbour list overflow after recount");
+    check_total();
   }
   throw std::runtime_error("find_ngb: slot overflow persisted after recount");
 }
@@ -489,6 +501,7 @@ void Engine<T>::hydro() {
 // own reactions
 template <class T>
 void Engine<T>::reaction() {
+  check_total();
   TimerScope ts(timers_, "reaction", s_);
   launch_reaction_pairs(ngb_offset_, ngb_count_, list_, pairs_, owner_, nd_, s_);
   launch_radix_sort(pairs_, pairs_alt_, ngb_tot_, bits_for(nd_), sort_tmp_, s_);
@@ -590,7 +603,7 @@ template <class T>
 void Engine<T>::sort_bodies() {
   refresh_if_dirty();
   TimerScope ts(timers_, "ph_sort", s_);
-  launch_ph_keys<T>(pos_, n_, p_.ph_bits, key_hi_, key_lo_, scan_tmp_, s_);
+  launch_ph_keys<T>(pos_, n_, p_.ph_bits, key_hi_, key_lo_, (int*)partials_.p, s_);
   // (key_lo, i) -> sort -> (key_hi[perm], perm) -> sort
   KV* kv = pairs_.p;
   KV* alt = pairs_alt_.p;
