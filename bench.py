@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-updates/s of the SPH step at N = 1,048,576 (two n=1.5
+polytropes on a parabolic encounter, 64 +- 8 neighbours, Plummer eps = 0.002),
+mixed-precision form vs the plain FP32 drop-in, with energy drift measured
+against an FP64 run of the same steps.
+
+`value` is the mixed form's whole-step throughput, N * K / wall(K steps), with
+the state resident in HBM (the per-step host kd-tree build and its transfers
+are inside the timed region, as in the reference's step()).
+
+Extra objects on the JSON line:
+  roofline      gravity kernel (the dominant kernel) of the mixed form:
+                20 flop per pair-interaction x N^2 per launch / mean launch
+                time from HIP events on the engine's stream, against MI355X's
+                157.3 TFLOP/s FP32 peak; traffic = HBM bytes per launch from
+                the committed rocprofv3 counter summary when present
+  cpu_baseline  the CPU oracle (reference arithmetic, OpenMP over particles)
+                on the host cores, on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP32_TFLOPS = 157.3
+FLOP_PER_PAIR = 20
+WORKLOAD = "two n=1.5 polytropes (R=1, M=0.5 each), parabolic orbit q=0.5 from separation 5; " \
+           "gas+radiation EOS mu=0.6 a=1e-3; 64+-8 neighbours; Plummer eps=0.002"
+
+
+def run_form(cs, form, n, seed, warmup, steps, timing=True):
+    e = cs.make_engine(form, 3, n, seed)
+    e.set_scalar("timing", 1 if timing else 0)
+    e.init()
+    energies = [e.stats()["E"]]
+    for _ in range(warmup):
+        e.step()
+        energies.append(e.stats()["E"])
+    e.reset_timers()
+    e.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.step()
+        energies.append(e.stats()["E"])
+    e.sync()
+    wall = time.perf_counter() - t0
+    timers = e.timers()
+    mom = e.stats()
+    out = {"wall_s": wall, "energies": energies, "timers": timers, "dt": e.scalar("dt"),
+           "momentum_residual": [mom["Fx"], mom["Fy"], mom["Fz"]]}
+    e.close()
+    return out
+
+
+def traffic_from_profile(kernel_prefix):
+    """HBM bytes per launch of the gravity kernel from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = ROOT / "profiles" / "gravity_mixed_c3_1M_pmc.txt"
+    if not path.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+-]+)", path.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(cs, n, seed, gravity_sample):
+    """Oracle FP32 step on the host: SPH passes and integration on all N, the
+    N^2 gravity on `gravity_sample` i-particles against all N j, scaled."""
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    p = cs.preset_params(3, n)
+    p.n_threads = threads
+    pos, vel, hyd = cs.generate_ic(3, n, seed)
+    o = oracle.Oracle(32, p)
+    o.load_ic(pos, vel, hyd)
+    t0 = time.perf_counter()
+    o.run_pass(oracle.SPH_ACC)
+    t_sph = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.gravity_range(0, gravity_sample)
+    t_grav = (time.perf_counter() - t0) * (p.n_dev / gravity_sample)
+    o.set_scalar("dt", 1e-4)
+    t0 = time.perf_counter()
+    o.run_pass(oracle.ADD_GRAVITY)
+    o.run_pass(oracle.TIMESTEP)
+    o.run_pass(oracle.PREDICT)
+    o.run_pass(oracle.CORRECT)
+    t_int = time.perf_counter() - t0
+    step = t_sph + t_grav + t_int
+    return {
+        "value": n / step, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+        "sample": (f"one plain-FP32 step of the same {n}-particle workload on {threads} threads: "
+                   f"tree+h-solve+lists+density+EOS+hydro+reaction on all particles ({t_sph:.2f} s), "
+                   f"direct gravity for {gravity_sample} i-particles against all j scaled by "
+                   f"{p.n_dev / gravity_sample:.0f} ({t_grav:.1f} s), integration ({t_int:.3f} s)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--seed", type=int, default=12345)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-gravity-sample", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import cudasph_amd as cs
+    n, K, W = args.n, args.steps, args.warmup
+    mixed = run_form(cs, "mixed", n, args.seed, W, K)
+    plain = None if args.no_plain else run_form(cs, "plain", n, args.seed, W, K)
+    fp64 = None if args.no_fp64 else run_form(cs, "fp64", n, args.seed, W, K, timing=False)
+
+    value = n * K / mixed["wall_s"]
+    g_ms, g_calls = mixed["timers"].get("gravity", (float("nan"), 0))
+    g_avg = g_ms / max(g_calls, 1) / 1e3
+    nd = cs.preset_params(3, n).n_dev
+    achieved = FLOP_PER_PAIR * float(nd) * float(nd) / g_avg / 1e12
+
+    def drift(run):
+        if run is None or fp64 is None:
+            return None
+        e0 = abs(fp64["energies"][0])
+        per = [abs(a - b) / e0 / max(k, 1) for k, (a, b) in enumerate(zip(run["energies"], fp64["energies"]))]
+        return {"max_per_step": max(per[1:]) if len(per) > 1 else 0.0,
+                "final": abs(run["energies"][-1] - fp64["energies"][-1]) / e0}
+
+    line = {
+        "metric": "particle-updates/s @1M (mixed vs plain FP32, rel err <=1e-6); adjoint/fwd ratio",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": mixed["wall_s"] / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded initial conditions generated in-process)",
+        "config": {"workload": WORKLOAD, "n": n, "form": "mixed", "seed": args.seed},
+        "plain_fp32": None if plain is None else {
+            "value": n * K / plain["wall_s"], "ms_per_step": plain["wall_s"] / K * 1e3,
+            "gravity_ms": plain["timers"].get("gravity", (0, 1))[0] / max(plain["timers"].get("gravity", (0, 1))[1], 1)},
+        "mixed_over_plain": None if plain is None else plain["wall_s"] / mixed["wall_s"],
+        "energy_drift_vs_fp64": {"definition": "|E_form(k) - E_fp64(k)| / |E_fp64(0)| / k over the run",
+                                 "mixed": drift(mixed), "plain": drift(plain)},
+        "kernel_ms_per_step": {k: v[0] / max(K, 1) for k, v in sorted(mixed["timers"].items())},
+        "roofline": {"bound": "valu", "kernel": "k_gravity_mixed_plummer",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profile("k_gravity"),
+                     "per_launch": f"{FLOP_PER_PAIR} flop x {nd}^2 pairs",
+                     "launch_ms": g_avg * 1e3},
+        "adjoint_over_forward": None,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(cs, n, args.seed, args.cpu_gravity_sample)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -94,6 +94,10 @@ def lib() -> C.CDLL:
     L.sph_state_file_count.argtypes = [cp]
     L.sph_state_file_count.restype = i
     L.sph_abi_set_params.argtypes = [C.POINTER(Params), i]
+    L.sph_build_kd_tree.argtypes = [i, i, i, vp, vp, vp]
+    L.sph_build_kd_tree.restype = i
+    L.sph_build_kd_tree64.argtypes = [i, i, i, vp, vp, vp]
+    L.sph_build_kd_tree64.restype = i
     _lib = L
     return L
 
@@ -120,6 +124,24 @@ def generate_ic(config: int, n: int, seed: int = 12345):
                              hyd.ctypes.data) != 0:
         raise ValueError(f"ic generation failed for config {config}")
     return pos, vel, hyd
+
+
+def build_kd_tree(pos4, n_dev: int | None = None, n_threads: int = 0):
+    """Host kd-tree (reference-exact) of pos4[:, :3]; returns (tree, particle_map)."""
+    pos4 = np.ascontiguousarray(pos4)
+    n = len(pos4)
+    nd = n if n_dev is None else int(n_dev)
+    hm = np.zeros((nd, 2), np.uint32)
+    if pos4.dtype == np.float64:
+        tree = np.zeros((nd + 2, 4), np.float64)
+        rc = lib().sph_build_kd_tree64(n, nd, n_threads, pos4.ctypes.data, tree.ctypes.data, hm.ctypes.data)
+    else:
+        pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+        tree = np.zeros((nd + 2, 4), np.float32)
+        rc = lib().sph_build_kd_tree(n, nd, n_threads, pos4.ctypes.data, tree.ctypes.data, hm.ctypes.data)
+    if rc != 0:
+        raise _err("sph_build_kd_tree")
+    return tree, hm
 
 
 def _err(what: str):

@@ -8,6 +8,7 @@
 
 #include "../../include/cudasph.h"
 #include "engine.h"
+#include "kdtree.h"
 
 using namespace cudasph;
 
@@ -175,6 +176,21 @@ int sph_engine_timers(void* e, const char** names, double* ms, long* calls, int 
 }
 void sph_engine_reset_timers(void* e) {
   guard([&] { E(e)->reset_timers(); return 0; }, 0);
+}
+
+int sph_build_kd_tree(int n, int n_dev, int n_threads, const sph_float4* pos, sph_float4* tree,
+                      sph_kv* particle_map) {
+  return guard([&] {
+    build_kd_tree<float>((const F4*)pos, n, n_dev, n_threads, (F4*)tree, (KV*)particle_map);
+    return 0;
+  }, -1);
+}
+
+int sph_build_kd_tree64(int n, int n_dev, int n_threads, const double* pos4, double* tree4, sph_kv* particle_map) {
+  return guard([&] {
+    build_kd_tree<double>((const D4*)pos4, n, n_dev, n_threads, (D4*)tree4, (KV*)particle_map);
+    return 0;
+  }, -1);
 }
 
 int sph_state_file_count(const char* path) {

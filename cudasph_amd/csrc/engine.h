@@ -81,6 +81,7 @@ struct HBuf {
 class Timers {
  public:
   void begin(const char* name, hipStream_t s);
+  void add_host(const char* name, double ms);
   void end(hipStream_t s);
   void resolve();
   int list(const char** names, double* ms, long* calls, int max_n);

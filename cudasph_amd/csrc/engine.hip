@@ -2,6 +2,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -34,6 +35,12 @@ void Timers::end(hipStream_t s) {
   CS_HIP(hipEventRecord(b, s));
   pending_.push_back({open_, open_ev_, b});
   open_ = nullptr;
+}
+void Timers::add_host(const char* name, double ms) {
+  if (!enabled) return;
+  auto& a = acc_[name];
+  a.first += ms;
+  a.second += 1;
 }
 void Timers::resolve() {
   for (auto& p : pending_) {
@@ -92,6 +99,7 @@ class Engine final : public EngineBase {
  public:
   Engine(int form, const sph_params& p, int device);
   ~Engine() override {
+    if (fetched_) (void)hipEventDestroy(fetched_);
     if (s_) (void)hipStreamDestroy(s_);
   }
   long set(const std::string& name, const void* host, long n_bytes) override;
@@ -141,6 +149,9 @@ class Engine final : public EngineBase {
   }
 
   void tree();
+  void tree_fetch();
+  void tree_build();
+  void sph_after_tree();
   void solve();
   void offsets();
   long check_total();
@@ -165,6 +176,7 @@ class Engine final : public EngineBase {
   int n_, nd_;
   long cap_;
   hipStream_t s_ = nullptr;
+  hipEvent_t fetched_ = nullptr;
   Timers timers_;
 
   DBuf<V4<T>> pos_, vel_, hydro_, dots_, bdots_, hydro2_, grav_, tree_, gpos_, gvel_, old_vu_, tmp4_;
@@ -198,6 +210,7 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
   if (form == SPH_FORM_MIXED && !kF32) throw std::invalid_argument("sph_engine: mixed form is FP32");
   CS_HIP(hipSetDevice(device));
   CS_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+  CS_HIP(hipEventCreateWithFlags(&fetched_, hipEventDisableTiming));
   cap_ = 100L * nd_;  // ngb_tot_max (read_state.cpp:29)
 
   for (auto* b : {&pos_, &vel_, &hydro_, &dots_, &bdots_, &hydro2_, &grav_, &gpos_, &gvel_, &old_vu_, &tmp4_})
@@ -383,17 +396,33 @@ void Engine<T>::stats(double* o) {
 // ---------------------------------------------------------------------------
 // passes
 
-// kd-tree build on the host (sph_accelerations.cpp:14-33), reference-exact
+// kd-tree build on the host (sph_accelerations.cpp:14-33), reference-exact.
+// tree_fetch enqueues the position download; tree_build waits for it, builds
+// the tree and enqueues the upload, so GPU work queued in between (gravity
+// under Plummer softening) runs while the host builds.
 template <class T>
-void Engine<T>::tree() {
-  TimerScope ts(timers_, "tree_build_host", s_);
+void Engine<T>::tree_fetch() {
   CS_HIP(hipMemcpyAsync(h_pos_.p, pos_, (size_t)nd_ * sizeof(V4<T>), hipMemcpyDeviceToHost, s_));
   CS_HIP(hipMemcpyAsync(&h_small_[0], dt_cur_, 8, hipMemcpyDeviceToHost, s_));
-  sync();
+  CS_HIP(hipEventRecord(fetched_, s_));
+}
+
+template <class T>
+void Engine<T>::tree_build() {
+  CS_HIP(hipEventSynchronize(fetched_));
   std::memcpy(&dt_, &h_small_[0], 8);
+  const auto t0 = std::chrono::steady_clock::now();
   build_kd_tree<T>(h_pos_.p, n_, nd_, p_.n_threads, h_tree_.p, h_hash_.p);
+  timers_.add_host("tree_build_host",
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   CS_HIP(hipMemcpyAsync(tree_, h_tree_.p, (size_t)(nd_ + 2) * sizeof(V4<T>), hipMemcpyHostToDevice, s_));
   CS_HIP(hipMemcpyAsync(hash_, h_hash_.p, (size_t)nd_ * sizeof(KV), hipMemcpyHostToDevice, s_));
+}
+
+template <class T>
+void Engine<T>::tree() {
+  tree_fetch();
+  tree_build();
 }
 
 template <class T>
@@ -520,6 +549,11 @@ void Engine<T>::reaction() {
 template <class T>
 void Engine<T>::sph_accelerations() {
   tree();
+  sph_after_tree();
+}
+
+template <class T>
+void Engine<T>::sph_after_tree() {
   solve();
   offsets();
   find();
@@ -654,8 +688,17 @@ void Engine<T>::step() {
   ++iteration_;
   if (p_.ph_period > 0 && iteration_ % p_.ph_period == 0) sort_bodies();
   predict();
-  sph_accelerations();   // reads back dt_ with the positions
-  gravity();
+  // Under Plummer softening gravity reads only the predicted positions and
+  // masses, so it is queued before the SPH passes and runs while the host
+  // builds the kd-tree; the results are the same as in the reference's order
+  // (sph_accelerations, then CUDA_gravity, step.cpp:65,94).  The spline
+  // softening needs the new ranges and keeps that order.
+  const bool grav_first = p_.grav_mode == 1;
+  tree_fetch();            // also reads back dt_
+  if (grav_first) gravity();
+  tree_build();
+  sph_after_tree();
+  if (!grav_first) gravity();
   add_gravity();
   global_time_ += dt_;
   stats_pass();

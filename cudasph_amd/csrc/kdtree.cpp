@@ -25,36 +25,6 @@ struct Item {
   T c[3];
 };
 
-// kd_tree.cpp:4-31.  Recursing into the smaller partition and iterating on
-// the larger one visits the same disjoint ranges with the same data, so the
-// result equals the doubly recursive original while the stack stays
-// logarithmic.
-template <class T>
-void quicksort_first_pivot(Item<T>* a, int axis, int left, int right) {
-  while (left < right) {
-    const int lo = left, hi = right;
-    const Item<T> pivot = a[left];
-    const T key = pivot.c[axis];
-    while (left < right) {
-      while (a[right].c[axis] >= key && left < right) --right;
-      if (left != right) a[left++] = a[right];
-      while (a[left].c[axis] <= key && left < right) ++left;
-      if (left != right) a[right--] = a[left];
-    }
-    a[left] = pivot;
-    const int mid = left;
-    if (mid - lo < hi - mid) {
-      if (lo < mid) quicksort_first_pivot(a, axis, lo, mid - 1);
-      left = mid + 1;
-      right = hi;
-    } else {
-      if (hi > mid) quicksort_first_pivot(a, axis, mid + 1, hi);
-      left = lo;
-      right = mid - 1;
-    }
-  }
-}
-
 // size of the left subtree of a left-balanced tree of n nodes (kd_tree.cpp:63-79)
 inline int left_size(int n) {
   int m = 1;
@@ -63,6 +33,8 @@ inline int left_size(int n) {
   const int last = n - (m - 1);
   return last <= m / 2 ? (m - 2) / 2 + last : (m - 2) / 2 + m / 2;
 }
+
+constexpr int kParallelGrain = 32768;   // elements below which work stays on the calling thread
 
 template <class T>
 struct Builder {
@@ -76,15 +48,50 @@ struct Builder {
     return false;
   }
 
+  // First-element-pivot quicksort, kd_tree.cpp:4-31.  Each partition step is
+  // the reference's; the two resulting ranges are disjoint, so sorting them
+  // in either order, or concurrently, gives the same array.  The smaller
+  // range is recursed into (on a spare thread when it is large) and the
+  // larger one is iterated on, which keeps the stack logarithmic.
+  void quicksort(Item<T>* a, int axis, int left, int right) {
+    std::vector<std::thread> spawned;
+    while (left < right) {
+      const int lo = left, hi = right;
+      const Item<T> pivot = a[left];
+      const T key = pivot.c[axis];
+      while (left < right) {
+        while (a[right].c[axis] >= key && left < right) --right;
+        if (left != right) a[left++] = a[right];
+        while (a[left].c[axis] <= key && left < right) ++left;
+        if (left != right) a[right--] = a[left];
+      }
+      a[left] = pivot;
+      const int mid = left;
+      int s_lo, s_hi;   // smaller side
+      if (mid - lo < hi - mid) {
+        s_lo = lo; s_hi = mid - 1; left = mid + 1; right = hi;
+      } else {
+        s_lo = mid + 1; s_hi = hi; left = lo; right = mid - 1;
+      }
+      if (s_lo < s_hi) {
+        if (s_hi - s_lo > kParallelGrain && take_thread())
+          spawned.emplace_back([=] { quicksort(a, axis, s_lo, s_hi); spare_threads.fetch_add(1); });
+        else
+          quicksort(a, axis, s_lo, s_hi);
+      }
+    }
+    for (auto& t : spawned) t.join();
+  }
+
   void build(int node, Item<T>* a, int depth, int n) {
     const int axis = depth % 3;
-    quicksort_first_pivot(a, axis, 0, n - 1);
+    quicksort(a, axis, 0, n - 1);
     const int lt = left_size(n);
     const Item<T>& e = a[lt];
     tree[node] = {e.c[0], e.c[1], e.c[2], host_node_word(T(0), e.id | (axis << 24))};
     const int nl = lt, nr = n - lt - 1;
     std::thread worker;
-    const bool spawned = nl > 32768 && nr > 0 && take_thread();
+    const bool spawned = nl > kParallelGrain && nr > 0 && take_thread();
     if (spawned) worker = std::thread([=] { build(2 * node, a, depth + 1, nl); });
     else if (nl > 0) build(2 * node, a, depth + 1, nl);
     if (nr > 0) build(2 * node + 1, a + lt + 1, depth + 1, nr);

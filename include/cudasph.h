@@ -87,6 +87,19 @@ int cuda_get_device(void);
  * same size.  Sorted output ends in pData0. */
 void RadixSort(sph_kv* pData0, sph_kv* pData1, unsigned int elements, unsigned int bits);
 
+/* kd_tree.cpp:97-128 with the packing of sph_accelerations.cpp:21-33 (host
+ * code, no GPU): the reference's left-balanced kd-tree of pos[0..n) (x,y,z
+ * used), built with the reference's quicksort so ties resolve identically,
+ * on up to n_threads host threads (0 = all).  tree receives n_dev+2 records
+ * in the device layout (w = int bits of id | split<<24; far sentinels past n;
+ * a zero record at n_dev+1); particle_map[t] = {t, particle at heap node t+1}
+ * (kd_tree.cpp:117-128), padding entries map to themselves.  Returns 0. */
+int sph_build_kd_tree(int n, int n_dev, int n_threads, const sph_float4* pos, sph_float4* tree,
+                      sph_kv* particle_map);
+/* the same on FP64 positions (the FP64 form's tree; w holds the int as a double) */
+int sph_build_kd_tree64(int n, int n_dev, int n_threads, const double* pos4, double* tree4,
+                        sph_kv* particle_map);
+
 /* No counterpart in the reference (its DESIRED_NGB, MAX_ITER, EOS unit
  * constants and M4 softening are compile-time constants, dev_solve_range.cu:14-15,
  * dev_compute_pressure.cu:55-81): sets the values the group-1 entry points use.

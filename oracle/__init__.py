@@ -48,6 +48,7 @@ def lib():
         L.orc_pass.argtypes = [vp, i]
         L.orc_pass.restype = i
         L.orc_stats.argtypes = [vp, C.POINTER(C.c_double)]
+        L.orc_gravity_range.argtypes = [vp, i, i]
         L.orc_kernel_fns.argtypes = [i, i, vp, vp]
         L.orc_pressure.argtypes = [i, vp, d, d, d]
         L.orc_pressure.restype = d
@@ -116,6 +117,9 @@ class Oracle:
     def step(self, n=1):
         for _ in range(n):
             self.run_pass(STEP)
+
+    def gravity_range(self, i0, i1):
+        lib().orc_gravity_range(self._h, int(i0), int(i1))
 
     def scalar(self, name):
         return lib().orc_get_scalar(self._h, name.encode())

@@ -535,6 +535,7 @@ struct Base {
   virtual long get(const std::string& name, void* dst, long max_bytes) = 0;
   virtual long set(const std::string& name, const void* src, long n_bytes) = 0;
   virtual int pass(int id) = 0;
+  virtual void gravity_range(int i0, int i1) = 0;
   double dt = 0, global_time = 0;
   int iteration = 0, n_overflow_reruns = 0, n_count_mismatch = 0, max_count = 0;
   long ngb_tot = 0;
@@ -918,6 +919,8 @@ template <class T> struct State : Base {
     correct(dt, dth);
   }
 
+  void gravity_range(int i0, int i1) override { gravity(i0, i1); }
+
   int pass(int id) override {
     switch (id) {
       case ORC_TREE: build_tree(); break;
@@ -978,6 +981,8 @@ void orc_set_scalar(void* h, const char* name, double v) {
   else if (s == "global_time") b->global_time = v;
   else if (s == "iteration") b->iteration = (int)v;
 }
+void orc_gravity_range(void* h, int i0, int i1) { static_cast<Base*>(h)->gravity_range(i0, i1); }
+
 void orc_stats(void* h, double* o) {
   const Stats& s = static_cast<Base*>(h)->st;
   o[0] = s.T; o[1] = s.U; o[2] = s.W; o[3] = s.E; o[4] = s.J;

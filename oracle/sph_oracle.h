@@ -59,6 +59,8 @@ long   orc_set(void* h, const char* name, const void* src, long n_bytes);
 double orc_get_scalar(void* h, const char* name);
 void   orc_set_scalar(void* h, const char* name, double v);
 int    orc_pass(void* h, int pass);
+/* gravity for i in [i0, i1) only (bounded CPU-baseline samples) */
+void   orc_gravity_range(void* h, int i0, int i1);
 /* statistics of the last ORC_STATS pass: T U W E |J| Fx Fy Fz Tx Ty Tz
  * (force and torque from the last ORC_ADD_GRAVITY) */
 void   orc_stats(void* h, double* out11);
