@@ -36,8 +36,10 @@ WORKLOAD = "two n=1.5 polytropes (R=1, M=0.5 each), parabolic orbit q=0.5 from s
            "gas+radiation EOS mu=0.6 a=1e-3; 64+-8 neighbours; Plummer eps=0.002"
 
 
-def run_form(cs, form, n, seed, warmup, steps, timing=True):
-    e = cs.make_engine(form, 3, n, seed)
+def run_form(cs, form, n, seed, warmup, steps, timing=True, i_order=0):
+    # the initial conditions are put in Peano-Hilbert order (the reference's
+    # periodic sort_bodies) so neighbour gathers read nearby memory
+    e = cs.make_engine(form, 3, n, seed, ph_sort=True, i_order=i_order)
     e.set_scalar("timing", 1 if timing else 0)
     e.init()
     energies = [e.stats()["E"]]
@@ -54,9 +56,36 @@ def run_form(cs, form, n, seed, warmup, steps, timing=True):
     wall = time.perf_counter() - t0
     timers = e.timers()
     mom = e.stats()
-    out = {"wall_s": wall, "energies": energies, "timers": timers, "dt": e.scalar("dt"),
+    kbar = float(e.get("ngb_count")[:n].mean())
+    out = {"wall_s": wall, "kbar": kbar, "energies": energies, "timers": timers, "dt": e.scalar("dt"),
            "momentum_residual": [mom["Fx"], mom["Fy"], mom["Fz"]]}
     e.close()
+    return out
+
+
+# compulsory HBM bytes per particle of each SPH pass (k = mean list length
+# including self): each state array element read or written once per pass,
+# lists read once per pass
+SPH_BYTES = {
+    "solve_range": lambda k: 60.0,        # hash 8 + pos R/W 32 + tree 16 + n_ngb 4
+    "find_ngb": lambda k: 48.0 + 4 * k,   # hash R/W 12 + pos 16 + offset 4 + tree 16 + list writes
+    "density": lambda k: 96.0 + 4 * k,    # hash 8 + pos, vel 32 + n, offset 8 + list + hydro R/W 32 + dots 16
+    "eos": lambda k: 32.0,                # hydro R/W
+    "hydro": lambda k: 128.0 + 8 * k,     # hash 8 + pos, vel, hydro, dots 64 + n, offset 8 + list + acc + 48
+    "reaction": lambda k: 56.0 + 8 * k,   # n, offset 8 + list + acc + pos 16 + bodies_dots R/W 32
+}
+
+
+def sph_bandwidth(timers, steps, n, kbar):
+    out = {}
+    for name, f in SPH_BYTES.items():
+        if name not in timers:
+            continue
+        ms, calls = timers[name]
+        per_launch_s = ms / max(calls, 1) / 1e3
+        gbps = f(kbar) * n / per_launch_s / 1e9
+        out[name] = {"bytes_per_particle": f(kbar), "ms": per_launch_s * 1e3, "GBps": gbps,
+                     "frac_of_8TBps": gbps / 8000.0}
     return out
 
 
@@ -114,7 +143,16 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-gravity-sample", type=int, default=256)
+    ap.add_argument("--i-order", type=int, default=0, help="0: memory order, 1: kd-tree heap order")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel over BYTES (for scaling memory counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import cudasph_amd as cs
+        ms = cs.lib().sph_copy_calibration(args.copy_calibration)
+        print(json.dumps({"copy_bytes": args.copy_calibration, "ms": ms,
+                          "GBps": 2 * args.copy_calibration / ms / 1e6}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -122,8 +160,8 @@ def main():
 
     import cudasph_amd as cs
     n, K, W = args.n, args.steps, args.warmup
-    mixed = run_form(cs, "mixed", n, args.seed, W, K)
-    plain = None if args.no_plain else run_form(cs, "plain", n, args.seed, W, K)
+    mixed = run_form(cs, "mixed", n, args.seed, W, K, i_order=args.i_order)
+    plain = None if args.no_plain else run_form(cs, "plain", n, args.seed, W, K, i_order=args.i_order)
     fp64 = None if args.no_fp64 else run_form(cs, "fp64", n, args.seed, W, K, timing=False)
 
     value = n * K / mixed["wall_s"]
@@ -153,7 +191,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded initial conditions generated in-process)",
-        "config": {"workload": WORKLOAD, "n": n, "form": "mixed", "seed": args.seed},
+        "config": {"workload": WORKLOAD, "n": n, "form": "mixed", "seed": args.seed,
+                   "initial_order": "Peano-Hilbert", "i_order": ["memory", "kd-tree heap"][args.i_order]},
         "plain_fp32": None if plain is None else {
             "value": n * K / plain["wall_s"], "ms_per_step": plain["wall_s"] / K * 1e3,
             "gravity_ms": plain["timers"].get("gravity", (0, 1))[0] / max(plain["timers"].get("gravity", (0, 1))[1], 1)},
@@ -161,6 +200,8 @@ def main():
         "energy_drift_vs_fp64": {"definition": "|E_form(k) - E_fp64(k)| / |E_fp64(0)| / k over the run",
                                  "mixed": drift(mixed), "plain": drift(plain)},
         "kernel_ms_per_step": {k: v[0] / max(K, 1) for k, v in sorted(mixed["timers"].items())},
+        "sph_hbm": {"mean_list_length": mixed["kbar"],
+                    "passes": sph_bandwidth(mixed["timers"], K, n, mixed["kbar"])},
         "roofline": {"bound": "valu", "kernel": "k_gravity_mixed_plummer",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profile("k_gravity"),

@@ -94,6 +94,8 @@ def lib() -> C.CDLL:
     L.sph_state_file_count.argtypes = [cp]
     L.sph_state_file_count.restype = i
     L.sph_abi_set_params.argtypes = [C.POINTER(Params), i]
+    L.sph_copy_calibration.argtypes = [l]
+    L.sph_copy_calibration.restype = d
     L.sph_build_kd_tree.argtypes = [i, i, i, vp, vp, vp]
     L.sph_build_kd_tree.restype = i
     L.sph_build_kd_tree64.argtypes = [i, i, i, vp, vp, vp]
@@ -275,8 +277,11 @@ class Engine:
 
 
 def make_engine(form, config: int, n: int | None = None, seed: int = 12345, params: Params | None = None,
-                device: int = 0) -> Engine:
-    """Engine loaded with the seeded initial conditions of a benchmark config."""
+                device: int = 0, ph_sort: bool = False, i_order: int | None = None) -> Engine:
+    """Engine loaded with the seeded initial conditions of a benchmark config.
+    ph_sort puts the particles in Peano-Hilbert order first (the reference's
+    sort_bodies, which it otherwise applies every 100 steps); i_order picks
+    the particle order of the per-particle kernels (0 memory, 1 kd-tree heap)."""
     p = params if params is not None else preset_params(config, n or 0)
     if (FORMS[form] if isinstance(form, str) else form) != FORM_PLAIN:
         p = p.copy()
@@ -284,4 +289,8 @@ def make_engine(form, config: int, n: int | None = None, seed: int = 12345, para
     e = Engine(form, p, device)
     pos, vel, hyd = generate_ic(config, p.n_bodies, seed)
     e.load_ic(pos, vel, hyd)
+    if ph_sort:
+        e.run_pass(PH_SORT)
+    if i_order is not None:
+        e.set_scalar("i_order", i_order)
     return e

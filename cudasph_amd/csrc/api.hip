@@ -9,6 +9,7 @@
 #include "../../include/cudasph.h"
 #include "engine.h"
 #include "kdtree.h"
+#include "kernels.h"
 
 using namespace cudasph;
 
@@ -191,6 +192,31 @@ int sph_build_kd_tree64(int n, int n_dev, int n_threads, const double* pos4, dou
     build_kd_tree<double>((const D4*)pos4, n, n_dev, n_threads, (D4*)tree4, (KV*)particle_map);
     return 0;
   }, -1);
+}
+
+double sph_copy_calibration(long bytes) {
+  return guard([&] {
+    const long n = bytes / 16;
+    F4 *a = nullptr, *b = nullptr;
+    CS_HIP(hipMalloc(&a, n * 16));
+    CS_HIP(hipMalloc(&b, n * 16));
+    CS_HIP(hipMemset(a, 0, n * 16));
+    hipEvent_t e0, e1;
+    CS_HIP(hipEventCreate(&e0));
+    CS_HIP(hipEventCreate(&e1));
+    launch_copy16(a, b, n, nullptr);
+    CS_HIP(hipEventRecord(e0, nullptr));
+    launch_copy16(a, b, n, nullptr);
+    CS_HIP(hipEventRecord(e1, nullptr));
+    CS_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    CS_HIP(hipEventDestroy(e0));
+    CS_HIP(hipEventDestroy(e1));
+    CS_HIP(hipFree(a));
+    CS_HIP(hipFree(b));
+    return (double)ms;
+  }, -1.0);
 }
 
 int sph_state_file_count(const char* path) {

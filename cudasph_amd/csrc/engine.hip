@@ -197,6 +197,7 @@ class Engine final : public EngineBase {
 
   double dt_ = 0, global_time_ = 0;
   int iteration_ = 0, exact_rsqrt_ = 0;
+  int i_order_ = 1;   // 1: lanes take particles in kd-tree heap order, 0: in memory order
   long ngb_tot_ = 0;
   int overflow_reruns_ = 0;
 };
@@ -352,6 +353,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "ngb_tot") return (double)ngb_tot_;
   if (name == "overflow_reruns") return overflow_reruns_;
   if (name == "exact_rsqrt") return exact_rsqrt_;
+  if (name == "i_order") return i_order_;
   if (name == "timing") return timers_.enabled;
   if (name == "cap") return (double)cap_;
   if (name == "real_bytes") return (double)sizeof(T);
@@ -371,6 +373,8 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     iteration_ = (int)v;
   } else if (name == "exact_rsqrt") {
     exact_rsqrt_ = (int)v;
+  } else if (name == "i_order") {
+    i_order_ = (int)v;
   } else if (name == "timing") {
     sync();
     timers_.enabled = v != 0;
@@ -413,6 +417,8 @@ void Engine<T>::tree_build() {
   std::memcpy(&dt_, &h_small_[0], 8);
   const auto t0 = std::chrono::steady_clock::now();
   build_kd_tree<T>(h_pos_.p, n_, nd_, p_.n_threads, h_tree_.p, h_hash_.p);
+  if (i_order_ == 0)   // lanes take particles in memory order (the reference's identity hash)
+    for (int t = 0; t < nd_; ++t) h_hash_.p[t] = {(uint32_t)t, (uint32_t)t};
   timers_.add_host("tree_build_host",
                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   CS_HIP(hipMemcpyAsync(tree_, h_tree_.p, (size_t)(nd_ + 2) * sizeof(V4<T>), hipMemcpyHostToDevice, s_));

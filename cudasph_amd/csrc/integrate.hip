@@ -485,6 +485,17 @@ template void launch_gather<D4>(const D4*, D4*, const KV*, int, hipStream_t);
 template void launch_gather<double>(const double*, double*, const KV*, int, hipStream_t);
 template void launch_gather<uint32_t>(const uint32_t*, uint32_t*, const KV*, int, hipStream_t);
 
+// plain 16-byte-per-lane copy: the reference point for scaling memory
+// counters (a copy of B bytes moves exactly B in and B out)
+__global__ __launch_bounds__(kBlock) void k_copy16(const F4* __restrict__ src, F4* __restrict__ dst, long n) {
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+void launch_copy16(const F4* src, F4* dst, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy16, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, src, dst, n);
+  CS_CHECK_LAUNCH();
+}
+
 __global__ __launch_bounds__(kBlock) void k_iota_kv(KV* kv, int n) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t < n) kv[t].value = (uint32_t)t;

@@ -116,5 +116,6 @@ long reduce_partials(int n);   // doubles needed for partial sums of n items (x6
 template <class R>
 void launch_gather(const R* src, R* dst, const KV* perm, int n, hipStream_t s);
 void launch_iota_kv(KV* kv, int n, hipStream_t s);
+void launch_copy16(const F4* src, F4* dst, long n, hipStream_t s);
 
 }  // namespace cudasph

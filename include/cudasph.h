@@ -183,6 +183,11 @@ void   sph_engine_reset_timers(void* e);
 
 /* Reference state file (read_state.cpp:5-67 / write_state.cpp:6-61): 124-byte
  * header + 23 doubles per particle. */
+/* Copies `bytes` between two fresh device buffers twice with a plain copy
+ * kernel (k_copy16) and returns the milliseconds of the second copy: the
+ * reference point for scaling the FETCH_SIZE / WRITE_SIZE counters of
+ * rocprofv3.  No counterpart in the reference. */
+double sph_copy_calibration(long bytes);
 int    sph_state_file_count(const char* path);   /* N from the header, or < 0 */
 int    sph_engine_read_state(void* e, const char* path);
 int    sph_engine_write_state(void* e, const char* path);
