@@ -132,6 +132,66 @@ def cpu_baseline(cs, n, seed, gravity_sample):
     }
 
 
+ADJ_WORKLOAD = "Plummer sphere (M=1, a=1, truncated at 10), 50+-5 neighbours, Plummer eps=0.005; " \
+               "loss = kinetic energy + sum rho^2 of the final state; gradient w.r.t. x0, v0, u0"
+
+
+def adjoint_leg(cs, n, steps, ckpt, fd_dirs, seed):
+    """Reverse mode of the FP64 form: wall time of the backward pass (with
+    checkpoint recomputation) against the plain FP64 forward of the same run,
+    and the gradient against central differences of the recorded map."""
+    import numpy as np
+    p = cs.preset_params(4, n)
+    pos, vel, hyd = cs.generate_ic(4, n, seed)
+
+    def engine():
+        e = cs.Engine("fp64", p)
+        e.load_ic(pos, vel, hyd)
+        e.sync()
+        return e
+
+    e0 = engine()
+    t0 = time.perf_counter()
+    e0.init()
+    e0.step(steps)
+    e0.sync()
+    t_fwd = time.perf_counter() - t0
+    e0.close()
+    e = engine()
+    t0 = time.perf_counter()
+    loss = e.adjoint_forward(steps, ckpt)
+    t_rec = time.perf_counter() - t0
+    e.set_scalar("timing", 1)
+    t0 = time.perf_counter()
+    gx, gv, gu = e.adjoint_backward()
+    t_bwd = time.perf_counter() - t0
+    timers = {k: v[0] for k, v in e.timers().items() if k.startswith("adjoint")}
+    e.set_scalar("timing", 0)
+    g = np.concatenate([gx.ravel(), gv.ravel(), gu])
+    z = np.concatenate([pos[:n, :3].ravel(), vel[:n, :3].ravel(), hyd[:n, 2]])
+
+    def L(zz):
+        return e.adjoint_replay(zz[: 3 * n].reshape(n, 3), zz[3 * n: 6 * n].reshape(n, 3), zz[6 * n:])
+
+    rs = np.random.default_rng(2024)
+    errs = []
+    for _ in range(fd_dirs):
+        d = rs.standard_normal(z.size)
+        d /= np.linalg.norm(d)
+        fd = (L(z + 1e-4 * d) - L(z - 1e-4 * d)) / 2e-4
+        errs.append(abs(g @ d - fd) / abs(fd))
+    e.close()
+    return {
+        "workload": ADJ_WORKLOAD, "n": n, "steps": steps, "ckpt_every": ckpt, "dtype": "f64",
+        "loss": loss, "forward_s": t_fwd, "recording_forward_s": t_rec, "backward_s": t_bwd,
+        "backward_kernel_ms": timers,
+        "adjoint_over_forward": t_bwd / t_fwd,
+        "fd_check": {"directions": fd_dirs, "step": 1e-4,
+                     "max_rel_err": max(errs) if errs else None,
+                     "median_rel_err": float(np.median(errs)) if errs else None},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +204,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-gravity-sample", type=int, default=256)
     ap.add_argument("--i-order", type=int, default=0, help="0: memory order, 1: kd-tree heap order")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-n", type=int, default=65536)
+    ap.add_argument("--adjoint-steps", type=int, default=20)
+    ap.add_argument("--adjoint-ckpt", type=int, default=5)
+    ap.add_argument("--fd-dirs", type=int, default=8,
+                    help="seeded unit directions for the finite-difference check of the gradient")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the reverse-mode leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (for scaling memory counters)")
     args = ap.parse_args()
@@ -160,6 +227,10 @@ def main():
 
     import cudasph_amd as cs
     n, K, W = args.n, args.steps, args.warmup
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(cs, args.adjoint_n, args.adjoint_steps, args.adjoint_ckpt, args.fd_dirs,
+                                     args.seed)))
+        return
     mixed = run_form(cs, "mixed", n, args.seed, W, K, i_order=args.i_order)
     plain = None if args.no_plain else run_form(cs, "plain", n, args.seed, W, K, i_order=args.i_order)
     fp64 = None if args.no_fp64 else run_form(cs, "fp64", n, args.seed, W, K, timing=False)
@@ -209,6 +280,10 @@ def main():
                      "launch_ms": g_avg * 1e3},
         "adjoint_over_forward": None,
     }
+    if not args.no_adjoint:
+        adj = adjoint_leg(cs, args.adjoint_n, args.adjoint_steps, args.adjoint_ckpt, args.fd_dirs, args.seed)
+        line["adjoint_over_forward"] = adj["adjoint_over_forward"]
+        line["adjoint"] = adj
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cs, n, args.seed, args.cpu_gravity_sample)
     print(json.dumps(line))

@@ -96,6 +96,12 @@ def lib() -> C.CDLL:
     L.sph_abi_set_params.argtypes = [C.POINTER(Params), i]
     L.sph_copy_calibration.argtypes = [l]
     L.sph_copy_calibration.restype = d
+    L.sph_adjoint_forward.argtypes = [vp, i, i, C.POINTER(C.c_double)]
+    L.sph_adjoint_forward.restype = i
+    L.sph_adjoint_backward.argtypes = [vp, vp, vp, vp]
+    L.sph_adjoint_backward.restype = i
+    L.sph_adjoint_replay.argtypes = [vp, vp, vp, vp, C.POINTER(C.c_double)]
+    L.sph_adjoint_replay.restype = i
     L.sph_build_kd_tree.argtypes = [i, i, i, vp, vp, vp]
     L.sph_build_kd_tree.restype = i
     L.sph_build_kd_tree64.argtypes = [i, i, i, vp, vp, vp]
@@ -266,6 +272,44 @@ class Engine:
 
     def reset_timers(self) -> None:
         lib().sph_engine_reset_timers(self._h)
+
+    # ---- reverse mode (FP64 form) ----------------------------------------
+    def adjoint_forward(self, steps: int, ckpt_every: int) -> float:
+        """Run init() + `steps` steps recording the map; returns the loss
+        sum m|v|^2/2 + sum rho^2 of the final state."""
+        loss = C.c_double()
+        if lib().sph_adjoint_forward(self._h, int(steps), int(ckpt_every), C.byref(loss)) != 0:
+            raise _err("adjoint_forward")
+        return loss.value
+
+    def adjoint_backward(self):
+        """Gradient of the recorded run's loss: (dL/dx0 [n,3], dL/dv0 [n,3], dL/du0 [n])."""
+        n = self.n
+        gx, gv, gu = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+        if lib().sph_adjoint_backward(self._h, gx.ctypes.data, gv.ctypes.data, gu.ctypes.data) != 0:
+            raise _err("adjoint_backward")
+        return gx, gv, gu
+
+    def adjoint_replay(self, x0=None, v0=None, u0=None) -> float:
+        """Loss of the recorded (frozen) map from another initial state."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (x0, v0, u0)]
+        ptrs = [None if a is None else a.ctypes.data for a in arrs]
+        loss = C.c_double()
+        if lib().sph_adjoint_replay(self._h, *ptrs, C.byref(loss)) != 0:
+            raise _err("adjoint_replay")
+        return loss.value
+
+    def adjoint_records(self):
+        """(ranges [K+1, n], timesteps [K+1]) of the recorded force evaluations."""
+        dts = np.zeros(4096)
+        got = lib().sph_engine_get(self._h, b"adj_dt", dts.ctypes.data, dts.nbytes)
+        if got < 0:
+            raise _err("get adj_dt")
+        dts = dts[: got // 8].copy()
+        rng = np.zeros((len(dts), self.nd))
+        if lib().sph_engine_get(self._h, b"adj_range", rng.ctypes.data, rng.nbytes) < 0:
+            raise _err("get adj_range")
+        return rng[:, : self.n], dts
 
     def write_state(self, path: str) -> None:
         if lib().sph_engine_write_state(self._h, str(path).encode()) != 0:

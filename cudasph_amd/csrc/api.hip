@@ -194,6 +194,29 @@ int sph_build_kd_tree64(int n, int n_dev, int n_threads, const double* pos4, dou
   }, -1);
 }
 
+int sph_adjoint_forward(void* e, int steps, int ckpt_every, double* loss) {
+  return guard([&] {
+    const double l = E(e)->adjoint_forward(steps, ckpt_every);
+    if (loss) *loss = l;
+    return 0;
+  }, -1);
+}
+
+int sph_adjoint_backward(void* e, double* grad_x, double* grad_v, double* grad_u) {
+  return guard([&] {
+    E(e)->adjoint_backward(grad_x, grad_v, grad_u);
+    return 0;
+  }, -1);
+}
+
+int sph_adjoint_replay(void* e, const double* x0, const double* v0, const double* u0, double* loss) {
+  return guard([&] {
+    const double l = E(e)->adjoint_replay(x0, v0, u0);
+    if (loss) *loss = l;
+    return 0;
+  }, -1);
+}
+
 double sph_copy_calibration(long bytes) {
   return guard([&] {
     const long n = bytes / 16;

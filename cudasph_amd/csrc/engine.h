@@ -32,6 +32,11 @@ class EngineBase {
   virtual int timers(const char** names, double* ms, long* calls, int max_n) = 0;
   virtual void reset_timers() = 0;
   virtual const sph_params& params() const = 0;
+  // reverse mode (FP64 form): record K steps, gradient of the loss with
+  // respect to the initial x, v, u, and re-runs of the recorded map
+  virtual double adjoint_forward(int steps, int ckpt_every) = 0;
+  virtual void adjoint_backward(double* gx, double* gv, double* gu) = 0;
+  virtual double adjoint_replay(const double* x0, const double* v0, const double* u0) = 0;
 };
 
 EngineBase* make_engine(int form, const sph_params& p, int device);

@@ -1,14 +1,5 @@
 // Device-resident SPH engine: the passes of step.cpp:13-147 over HBM state.
-#include "engine.h"
-
-#include <algorithm>
-#include <chrono>
-#include <cmath>
-#include <cstring>
-
-#include "kdtree.h"
-#include "kernels.h"
-#include "sph_math.h"
+#include "engine_impl.h"
 
 namespace cudasph {
 
@@ -79,128 +70,6 @@ Timers::~Timers() {
   }
   for (auto e : pool_) (void)hipEventDestroy(e);
 }
-
-struct TimerScope {
-  Timers& t;
-  hipStream_t s;
-  TimerScope(Timers& t_, const char* name, hipStream_t s_) : t(t_), s(s_) { t.begin(name, s); }
-  ~TimerScope() { t.end(s); }
-};
-
-inline int bits_for(long v) {
-  int b = 0;
-  while ((1L << b) <= v) ++b;
-  return b;
-}
-
-// ---------------------------------------------------------------------------
-template <class T>
-class Engine final : public EngineBase {
- public:
-  Engine(int form, const sph_params& p, int device);
-  ~Engine() override {
-    if (fetched_) (void)hipEventDestroy(fetched_);
-    if (s_) (void)hipStreamDestroy(s_);
-  }
-  long set(const std::string& name, const void* host, long n_bytes) override;
-  long get(const std::string& name, void* host, long max_bytes) override;
-  double get_scalar(const std::string& name) override;
-  void set_scalar(const std::string& name, double v) override;
-  void pass(int id) override;
-  void run(int steps) override {
-    for (int k = 0; k < steps; ++k) step();
-  }
-  void sync() override { CS_HIP(hipStreamSynchronize(s_)); }
-  void stats(double* out11) override;
-  int timers(const char** names, double* ms, long* calls, int max_n) override {
-    sync();
-    return timers_.list(names, ms, calls, max_n);
-  }
-  void reset_timers() override {
-    sync();
-    timers_.reset();
-  }
-  const sph_params& params() const override { return p_; }
-
- private:
-  static constexpr bool kF32 = sizeof(T) == 4;
-  bool mixed() const { return form_ == SPH_FORM_MIXED; }
-  WalkParams walk_params() const {
-    return {p_.n_bodies, p_.fix_phantom, p_.rel_err, p_.desired_ngb, p_.max_iter};
-  }
-  EosConst eos_const() const {
-    EosConst c;
-    c.density_unit = p_.eos_density_unit;
-    c.energy_unit = p_.eos_energy_unit;
-    c.pressure_unit = p_.eos_pressure_unit;
-    c.amu = p_.eos_amu;
-    c.boltzmann = p_.eos_boltzmann;
-    c.rad_const = p_.eos_rad_const;
-    c.gamma = p_.eos_gamma;
-    c.temp_base = p_.eos_temp_base;
-    c.newton = p_.eos_newton;
-    return c;
-  }
-  void refresh_if_dirty() {
-    if (mixed() && dirty64_) {
-      launch_refresh_mixed(pos64_, vel64_, u64_, (F4*)pos_.p, pos_lo_, (F4*)vel_.p, (F4*)hydro_.p, n_, s_);
-      dirty64_ = false;
-    }
-  }
-
-  void tree();
-  void tree_fetch();
-  void tree_build();
-  void sph_after_tree();
-  void solve();
-  void offsets();
-  long check_total();
-  void find();
-  void density();
-  void eos();
-  void hydro();
-  void reaction();
-  void sph_accelerations();
-  void gravity();
-  void add_gravity();
-  void timestep(double* out);
-  void predict();
-  void correct(const double* dt_new, const double* dt_old);
-  void stats_pass();
-  void sort_bodies();
-  void init();
-  void step();
-
-  int form_;
-  sph_params p_;
-  int n_, nd_;
-  long cap_;
-  hipStream_t s_ = nullptr;
-  hipEvent_t fetched_ = nullptr;
-  Timers timers_;
-
-  DBuf<V4<T>> pos_, vel_, hydro_, dots_, bdots_, hydro2_, grav_, tree_, gpos_, gvel_, old_vu_, tmp4_;
-  DBuf<KV> hash_, pairs_, pairs_alt_, perm_;
-  DBuf<int> n_ngb_, ngb_count_, ngb_offset_, list_, owner_, tr_start_, tr_end_, flags_, scan_tmp_, sort_tmp_;
-  DBuf<T> acc_;
-  DBuf<double> dt_cur_, dt_next_, partials_, force6_, stats6_;
-  DBuf<uint32_t> key_hi_, key_lo_;
-  // mixed form
-  DBuf<D4> pos64_, vel64_, old64_;
-  DBuf<double> u64_;
-  DBuf<F4> pos_lo_, ghi_, glo_, gvel32_;
-  bool dirty64_ = false;
-
-  HBuf<V4<T>> h_pos_, h_tree_;
-  HBuf<KV> h_hash_;
-  HBuf<long long> h_small_;
-
-  double dt_ = 0, global_time_ = 0;
-  int iteration_ = 0, exact_rsqrt_ = 0;
-  int i_order_ = 1;   // 1: lanes take particles in kd-tree heap order, 0: in memory order
-  long ngb_tot_ = 0;
-  int overflow_reruns_ = 0;
-};
 
 template <class T>
 Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p) {
@@ -332,6 +201,12 @@ long Engine<T>::get(const std::string& name, void* host, long mb) {
   if (name == "ngb_offset") return take(ngb_offset_, ngb_offset_.bytes());
   if (name == "ngb_list") return take(list_, list_.bytes());
   if (name == "ngb_acc") return take(acc_, acc_.bytes());
+  if (name == "adj_range") return take(adj_range_, adj_range_.bytes());
+  if (name == "adj_dt") {
+    const size_t c = std::min<size_t>(adj_dt_.size() * 8, (size_t)mb);
+    std::memcpy(host, adj_dt_.data(), c);
+    return (long)c;
+  }
   if (mixed()) {
     if (name == "pos64") return take(pos64_, pos64_.bytes());
     if (name == "vel64") return take(vel64_, vel64_.bytes());
@@ -433,8 +308,20 @@ void Engine<T>::tree() {
 
 template <class T>
 void Engine<T>::solve() {
-  TimerScope ts(timers_, "solve_range", s_);
-  launch_solve_range<T>(pos_, n_ngb_, hash_, tree_, nd_, walk_params(), s_);
+  const size_t rec = (size_t)adj_f_ * nd_;
+  if (adj_mode_ >= kAdjReplay) {   // the recorded ranges and counts of this evaluation
+    launch_set_range<T>(pos_, adj_range_.p + rec, nd_, s_);
+    CS_HIP(hipMemcpyAsync(n_ngb_, adj_nngb_.p + rec, (size_t)nd_ * 4, hipMemcpyDeviceToDevice, s_));
+    return;
+  }
+  {
+    TimerScope ts(timers_, "solve_range", s_);
+    launch_solve_range<T>(pos_, n_ngb_, hash_, tree_, nd_, walk_params(), s_);
+  }
+  if (adj_mode_ == kAdjRecord) {
+    launch_get_range<T>(pos_, adj_range_.p + rec, nd_, s_);
+    CS_HIP(hipMemcpyAsync(adj_nngb_.p + rec, n_ngb_, (size_t)nd_ * 4, hipMemcpyDeviceToDevice, s_));
+  }
 }
 
 // list slot layout (sph_accelerations.cpp:58-72); the total is checked
@@ -534,15 +421,22 @@ void Engine<T>::hydro() {
 // slots by target j (stable, so each j sees its sources in ascending slot =
 // ascending i order, the reference's serial order) and let each j sum its
 // own reactions
+// list slots sorted by neighbour id (stable): for each j, the slots k whose
+// neighbour is j, in ascending k
 template <class T>
-void Engine<T>::reaction() {
+void Engine<T>::build_transpose() {
   check_total();
-  TimerScope ts(timers_, "reaction", s_);
   launch_reaction_pairs(ngb_offset_, ngb_count_, list_, pairs_, owner_, nd_, s_);
   launch_radix_sort(pairs_, pairs_alt_, ngb_tot_, bits_for(nd_), sort_tmp_, s_);
   CS_HIP(hipMemsetAsync(tr_start_, 0, tr_start_.bytes(), s_));
   CS_HIP(hipMemsetAsync(tr_end_, 0, tr_end_.bytes(), s_));
   launch_reaction_bounds(pairs_, ngb_tot_, nd_, tr_start_, tr_end_, s_);
+}
+
+template <class T>
+void Engine<T>::reaction() {
+  TimerScope ts(timers_, "reaction", s_);
+  build_transpose();
   if constexpr (kF32) {
     if (mixed()) {
       launch_reaction_mixed(pos_, pos_lo_, bdots_, acc_, pairs_, owner_, tr_start_, tr_end_, n_, s_);
@@ -573,6 +467,19 @@ void Engine<T>::sph_after_tree() {
 template <class T>
 void Engine<T>::gravity() {
   refresh_if_dirty();
+  if (adj_mode_ == kAdjReplayStoredGravity) {
+    CS_HIP(hipMemcpyAsync(grav_, adj_grav_.p + (size_t)adj_f_ * nd_, (size_t)nd_ * sizeof(V4<T>),
+                          hipMemcpyDeviceToDevice, s_));
+    return;
+  }
+  gravity_compute();
+  if (adj_mode_ == kAdjRecord)
+    CS_HIP(hipMemcpyAsync(adj_grav_.p + (size_t)adj_f_ * nd_, grav_, (size_t)nd_ * sizeof(V4<T>),
+                          hipMemcpyDeviceToDevice, s_));
+}
+
+template <class T>
+void Engine<T>::gravity_compute() {
   if constexpr (kF32) {
     if (mixed()) {
       launch_gravity_pack_mixed(pos64_, pos_, vel_, ghi_, glo_, gvel32_, n_, nd_, s_);
@@ -593,6 +500,20 @@ void Engine<T>::add_gravity() {
 
 template <class T>
 void Engine<T>::timestep(double* out) {
+  if (adj_mode_ >= kAdjReplay) {   // the recorded timestep of this evaluation
+    CS_HIP(hipMemcpyAsync(out, &adj_dt_[adj_f_], 8, hipMemcpyHostToDevice, s_));
+    sync();
+    return;
+  }
+  timestep_compute(out);
+  if (adj_mode_ == kAdjRecord) {
+    sync();
+    CS_HIP(hipMemcpy(&adj_dt_[adj_f_], out, 8, hipMemcpyDeviceToHost));
+  }
+}
+
+template <class T>
+void Engine<T>::timestep_compute(double* out) {
   if constexpr (kF32) {
     if (mixed()) {
       launch_timestep_mixed(pos_, u64_, hydro2_, bdots_, n_, p_.cn_vel, p_.cn_eth, partials_, out, s_);
@@ -681,6 +602,7 @@ void Engine<T>::sort_bodies() {
 
 template <class T>
 void Engine<T>::init() {
+  adj_f_ = 0;
   sph_accelerations();
   gravity();
   add_gravity();
@@ -692,7 +614,10 @@ void Engine<T>::init() {
 template <class T>
 void Engine<T>::step() {
   ++iteration_;
-  if (p_.ph_period > 0 && iteration_ % p_.ph_period == 0) sort_bodies();
+  adj_f_ = iteration_ - adj_iter0_;
+  // no re-sort while recording or replaying an adjoint trajectory (the
+  // records are indexed by particle)
+  if (p_.ph_period > 0 && iteration_ % p_.ph_period == 0 && adj_mode_ == kAdjOff) sort_bodies();
   predict();
   // Under Plummer softening gravity reads only the predicted positions and
   // masses, so it is queued before the SPH passes and runs while the host
@@ -737,6 +662,9 @@ void Engine<T>::pass(int id) {
     default: throw std::invalid_argument("unknown pass id");
   }
 }
+
+template class Engine<float>;
+template class Engine<double>;
 
 EngineBase* make_engine(int form, const sph_params& p, int device) {
   if (form == SPH_FORM_PLAIN || form == SPH_FORM_MIXED) return new Engine<float>(form, p, device);

@@ -1,0 +1,170 @@
+// Engine<T>: the device-resident simulation's class definition, shared by
+// engine.hip (forward passes) and adjoint.hip (reverse passes).
+#pragma once
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine.h"
+#include "kdtree.h"
+#include "kernels.h"
+#include "sph_math.h"
+
+namespace cudasph {
+
+struct TimerScope {
+  Timers& t;
+  hipStream_t s;
+  TimerScope(Timers& t_, const char* name, hipStream_t s_) : t(t_), s(s_) { t.begin(name, s); }
+  ~TimerScope() { t.end(s); }
+};
+
+inline int bits_for(long v) {
+  int b = 0;
+  while ((1L << b) <= v) ++b;
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+template <class T>
+class Engine final : public EngineBase {
+ public:
+  Engine(int form, const sph_params& p, int device);
+  ~Engine() override {
+    if (fetched_) (void)hipEventDestroy(fetched_);
+    if (s_) (void)hipStreamDestroy(s_);
+  }
+  long set(const std::string& name, const void* host, long n_bytes) override;
+  long get(const std::string& name, void* host, long max_bytes) override;
+  double get_scalar(const std::string& name) override;
+  void set_scalar(const std::string& name, double v) override;
+  void pass(int id) override;
+  void run(int steps) override {
+    for (int k = 0; k < steps; ++k) step();
+  }
+  void sync() override { CS_HIP(hipStreamSynchronize(s_)); }
+  void stats(double* out11) override;
+  int timers(const char** names, double* ms, long* calls, int max_n) override {
+    sync();
+    return timers_.list(names, ms, calls, max_n);
+  }
+  void reset_timers() override {
+    sync();
+    timers_.reset();
+  }
+  const sph_params& params() const override { return p_; }
+
+ private:
+  static constexpr bool kF32 = sizeof(T) == 4;
+  bool mixed() const { return form_ == SPH_FORM_MIXED; }
+  WalkParams walk_params() const {
+    return {p_.n_bodies, p_.fix_phantom, p_.rel_err, p_.desired_ngb, p_.max_iter};
+  }
+  EosConst eos_const() const {
+    EosConst c;
+    c.density_unit = p_.eos_density_unit;
+    c.energy_unit = p_.eos_energy_unit;
+    c.pressure_unit = p_.eos_pressure_unit;
+    c.amu = p_.eos_amu;
+    c.boltzmann = p_.eos_boltzmann;
+    c.rad_const = p_.eos_rad_const;
+    c.gamma = p_.eos_gamma;
+    c.temp_base = p_.eos_temp_base;
+    c.newton = p_.eos_newton;
+    return c;
+  }
+  void refresh_if_dirty() {
+    if (mixed() && dirty64_) {
+      launch_refresh_mixed(pos64_, vel64_, u64_, (F4*)pos_.p, pos_lo_, (F4*)vel_.p, (F4*)hydro_.p, n_, s_);
+      dirty64_ = false;
+    }
+  }
+
+  void tree();
+  void tree_fetch();
+  void tree_build();
+  void sph_after_tree();
+  void solve();
+  void offsets();
+  long check_total();
+  void find();
+  void density();
+  void eos();
+  void hydro();
+  void reaction();
+  void sph_accelerations();
+  void gravity();
+  void gravity_compute();
+  void add_gravity();
+  void timestep(double* out);
+  void timestep_compute(double* out);
+  void predict();
+  void correct(const double* dt_new, const double* dt_old);
+  void stats_pass();
+  void sort_bodies();
+  void init();
+  void step();
+  void build_transpose();
+
+ public:
+  // reverse mode of the FP64 form (adjoint.hip)
+  double adjoint_forward(int steps, int ckpt_every) override;
+  void adjoint_backward(double* gx, double* gv, double* gu) override;
+  double adjoint_replay(const double* x0, const double* v0, const double* u0) override;
+
+ private:
+  enum { kAdjOff = 0, kAdjRecord = 1, kAdjReplay = 2, kAdjReplayStoredGravity = 3 };
+  void adj_forces_backward(bool seed_rho);
+  void adj_save(DBuf<V4<T>>* dst[4], long slot);
+  void adj_load(DBuf<V4<T>>* src[4], long slot);
+  double adj_loss();
+  int adj_mode_ = kAdjOff, adj_K_ = 0, adj_k_ = 1, adj_f_ = 0, adj_iter0_ = 0;
+  bool adj_ready_ = false;
+  std::vector<double> adj_dt_;                       // timestep set by force evaluation f = 0..K
+  DBuf<double> adj_range_;                           // ranges of evaluation f, (K+1) x n_dev
+  DBuf<int> adj_nngb_;                               // h-solve counts of evaluation f
+  DBuf<V4<T>> adj_grav_;                             // gravity of evaluation f
+  DBuf<V4<T>> adj_ck_pos_, adj_ck_vel_, adj_ck_hyd_, adj_ck_bd_;     // checkpoints, every k steps
+  DBuf<V4<T>> adj_seg_pos_, adj_seg_vel_, adj_seg_hyd_, adj_seg_bd_; // pre-step states of one segment
+  DBuf<V4<T>> adj_init_pos_, adj_init_vel_, adj_init_hyd_, adj_final_vel_;
+  DBuf<V4<T>> ax_, av_, aa_;       // carried adjoints of the state: x, v, (a, udot)
+  DBuf<double> au_;                // carried adjoint of u
+  DBuf<V4<T>> xs_, vs_, hb1_, hb2_;  // one evaluation: x, v, (rho, P, u), (divv, curl, Omega, Psi)
+  DBuf<double> adj_out_, adj_partial_;
+  DBuf<double> pairbuf_;           // per-slot j-side adjoint contributions, 10 doubles each
+
+  int form_;
+  sph_params p_;
+  int n_, nd_;
+  long cap_;
+  hipStream_t s_ = nullptr;
+  hipEvent_t fetched_ = nullptr;
+  Timers timers_;
+
+  DBuf<V4<T>> pos_, vel_, hydro_, dots_, bdots_, hydro2_, grav_, tree_, gpos_, gvel_, old_vu_, tmp4_;
+  DBuf<KV> hash_, pairs_, pairs_alt_, perm_;
+  DBuf<int> n_ngb_, ngb_count_, ngb_offset_, list_, owner_, tr_start_, tr_end_, flags_, scan_tmp_, sort_tmp_;
+  DBuf<T> acc_;
+  DBuf<double> dt_cur_, dt_next_, partials_, force6_, stats6_;
+  DBuf<uint32_t> key_hi_, key_lo_;
+  // mixed form
+  DBuf<D4> pos64_, vel64_, old64_;
+  DBuf<double> u64_;
+  DBuf<F4> pos_lo_, ghi_, glo_, gvel32_;
+  bool dirty64_ = false;
+
+  HBuf<V4<T>> h_pos_, h_tree_;
+  HBuf<KV> h_hash_;
+  HBuf<long long> h_small_;
+
+  double dt_ = 0, global_time_ = 0;
+  int iteration_ = 0, exact_rsqrt_ = 0;
+  int i_order_ = 1;   // 1: lanes take particles in kd-tree heap order, 0: in memory order
+  long ngb_tot_ = 0;
+  int overflow_reruns_ = 0;
+};
+
+}  // namespace cudasph

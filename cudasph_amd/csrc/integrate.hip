@@ -496,6 +496,31 @@ void launch_copy16(const F4* src, F4* dst, long n, hipStream_t s) {
   CS_CHECK_LAUNCH();
 }
 
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_set_range(V4<T>* __restrict__ pos, const double* __restrict__ r, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) pos[i].w = T(r[i]);
+}
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_get_range(const V4<T>* __restrict__ pos, double* __restrict__ r, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) r[i] = (double)pos[i].w;
+}
+template <class T>
+void launch_set_range(V4<T>* pos, const double* range, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_range<T>, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, pos, range, n);
+  CS_CHECK_LAUNCH();
+}
+template <class T>
+void launch_get_range(const V4<T>* pos, double* range, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_get_range<T>, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, pos, range, n);
+  CS_CHECK_LAUNCH();
+}
+template void launch_set_range<float>(F4*, const double*, int, hipStream_t);
+template void launch_set_range<double>(D4*, const double*, int, hipStream_t);
+template void launch_get_range<float>(const F4*, double*, int, hipStream_t);
+template void launch_get_range<double>(const D4*, double*, int, hipStream_t);
+
 __global__ __launch_bounds__(kBlock) void k_iota_kv(KV* kv, int n) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t < n) kv[t].value = (uint32_t)t;

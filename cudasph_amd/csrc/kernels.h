@@ -116,6 +116,11 @@ long reduce_partials(int n);   // doubles needed for partial sums of n items (x6
 template <class R>
 void launch_gather(const R* src, R* dst, const KV* perm, int n, hipStream_t s);
 void launch_iota_kv(KV* kv, int n, hipStream_t s);
+// pos[i].w <-> range[i]
+template <class T>
+void launch_set_range(V4<T>* pos, const double* range, int n, hipStream_t s);
+template <class T>
+void launch_get_range(const V4<T>* pos, double* range, int n, hipStream_t s);
 void launch_copy16(const F4* src, F4* dst, long n, hipStream_t s);
 
 }  // namespace cudasph

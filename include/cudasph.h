@@ -181,16 +181,38 @@ int    sph_engine_stats(void* e, double* out11);
 int    sph_engine_timers(void* e, const char** names, double* ms, long* calls, int max_n);
 void   sph_engine_reset_timers(void* e);
 
+/* Reverse mode of the FP64 form (form SPH_FORM_FP64, Plummer gravity).  No
+ * counterpart in the reference.  The differentiated map runs init() and
+ * `steps` step()s from the engine's current state and returns
+ *     L = sum_i m_i |v_i|^2 / 2 + sum_i rho_i^2      (final state, real particles)
+ * with the smoothing ranges, h-solve counts and timesteps of every force
+ * evaluation held at the values of the recorded run.
+ *  sph_adjoint_forward:  runs and records the map (state checkpointed every
+ *                        `ckpt_every` steps, per-evaluation gravity stored);
+ *                        *loss receives L.
+ *  sph_adjoint_backward: dL/dx0, dL/dv0 (n x 3 doubles each) and dL/du0 (n
+ *                        doubles) of the recorded run into host arrays (any may
+ *                        be NULL); recomputes each checkpoint segment.
+ *  sph_adjoint_replay:   L of the recorded (frozen) map from another initial
+ *                        state: x0, v0 (n x 3) and u0 (n) host arrays, NULL =
+ *                        the recorded run's own.  Leaves the engine in the
+ *                        replayed final state.
+ * All return 0, or -1 with sph_last_error() set. */
+int    sph_adjoint_forward(void* e, int steps, int ckpt_every, double* loss);
+int    sph_adjoint_backward(void* e, double* grad_x, double* grad_v, double* grad_u);
+int    sph_adjoint_replay(void* e, const double* x0, const double* v0, const double* u0, double* loss);
+
 /* Reference state file (read_state.cpp:5-67 / write_state.cpp:6-61): 124-byte
  * header + 23 doubles per particle. */
+int    sph_state_file_count(const char* path);   /* N from the header, or < 0 */
+int    sph_engine_read_state(void* e, const char* path);
+int    sph_engine_write_state(void* e, const char* path);
+
 /* Copies `bytes` between two fresh device buffers twice with a plain copy
  * kernel (k_copy16) and returns the milliseconds of the second copy: the
  * reference point for scaling the FETCH_SIZE / WRITE_SIZE counters of
  * rocprofv3.  No counterpart in the reference. */
 double sph_copy_calibration(long bytes);
-int    sph_state_file_count(const char* path);   /* N from the header, or < 0 */
-int    sph_engine_read_state(void* e, const char* path);
-int    sph_engine_write_state(void* e, const char* path);
 
 #ifdef __cplusplus
 }
