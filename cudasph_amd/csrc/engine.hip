@@ -229,6 +229,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "overflow_reruns") return overflow_reruns_;
   if (name == "exact_rsqrt") return exact_rsqrt_;
   if (name == "i_order") return i_order_;
+  if (name == "grav_split_only") return grav_split_only_;
   if (name == "timing") return timers_.enabled;
   if (name == "cap") return (double)cap_;
   if (name == "real_bytes") return (double)sizeof(T);
@@ -250,6 +251,8 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     exact_rsqrt_ = (int)v;
   } else if (name == "i_order") {
     i_order_ = (int)v;
+  } else if (name == "grav_split_only") {
+    grav_split_only_ = (int)v;
   } else if (name == "timing") {
     sync();
     timers_.enabled = v != 0;
@@ -484,7 +487,8 @@ void Engine<T>::gravity_compute() {
     if (mixed()) {
       launch_gravity_pack_mixed(pos64_, pos_, vel_, ghi_, glo_, gvel32_, n_, nd_, s_);
       TimerScope ts(timers_, "gravity", s_);
-      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, nd_, nd_, p_.grav_mode, p_.grav_eps, s_);
+      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, nd_, nd_, p_.grav_mode, p_.grav_eps, grav_split_only_,
+                           s_);
       return;
     }
   }

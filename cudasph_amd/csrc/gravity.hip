@@ -192,37 +192,102 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
 
 // mixed form, Plummer: split positions, FMA pair arithmetic, blocked
 // compensated sums.  hi = (x,y,z,m) rounded, lo = (x-hi, y-hi, z-hi, 0).
-// Per pair: 3+3 subtractions and 3 additions for r, 3 FMAs for r^2+eps^2,
-// one rsq, 3 multiplies, 3 FMAs into the partial acceleration, 1 subtraction
-// for the potential.
+//
+// Two ways to form r = x_i - x_j, chosen per (block, j-tile):
+//  split:  r = (hi_i - hi_j) + (lo_i - lo_j), 9 operations per pair, relative
+//          error ~2^-24 for every pair;
+//  tile:   the tile is expressed relative to its middle record c,
+//          d_j = (hi_j - c) + lo_j (staged in LDS) and d_i = (hi_i - c) + lo_i
+//          (once per tile per lane), r = d_i - d_j: 3 operations per pair,
+//          absolute error <= 2^-24 (|d_i| + |d_j|) + rounding of r.
+// The tile form is used only when every lane of the block has
+// |d_i| >= 3 R_t, R_t = max_j |d_j|: then |r| >= |d_i| - R_t >= 2 R_t and
+// (|d_i| + |d_j|) / |r| <= (|d_i| + R_t) / (|d_i| - R_t) <= 2, so r keeps a
+// relative error of a few 2^-24 -- the accuracy of the split form.  Blocks
+// near a tile (the pairs whose separation the split coordinates exist for)
+// use the split form.  With particles in Peano-Hilbert order the tiles are
+// compact and most (block, tile) combinations are far apart; with an
+// unordered input the tiles are as large as the system and every block uses
+// the split form.  split_only = 1 forces the split form everywhere.
+// After r: 3 FMAs for r^2 + eps^2, one rsq, 3 multiplies, 3 FMAs into the
+// partial acceleration, 1 subtraction for the potential.
 __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* __restrict__ hi,
                                                                        const F4* __restrict__ lo,
                                                                        F4* __restrict__ out, int n_i, int n_j,
-                                                                       float eps2) {
-  __shared__ F4 s_hi[kJTile], s_lo[kJTile];
+                                                                       float eps2, int split_only) {
+  __shared__ F4 s_hi[kJTile], s_lo[kJTile], s_rel[kJTile];
+  __shared__ float s_red[2 * (kGravBlock / 64)];
   const int i = blockIdx.x * kGravBlock + threadIdx.x;
   const int ii = i < n_i ? i : n_i - 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const F4 pi = hi[ii], li = lo[ii];
   GAcc<float, 1> a[4];
   for (int t0 = 0; t0 < n_j; t0 += kJTile) {
-    const int cnt = stage_tile(s_hi, hi, t0, n_j);
-    for (int q = threadIdx.x; q < cnt; q += kGravBlock) s_lo[q] = lo[t0 + q];
+    const int cnt = min(kJTile, n_j - t0);
+    const F4 c = hi[t0 + cnt / 2];   // the middle record: about half the tile's extent from either end
     __syncthreads();
+    float r2max = 0.f;
+    for (int q = threadIdx.x; q < cnt; q += kGravBlock) {
+      const F4 h = hi[t0 + q], l = lo[t0 + q];
+      const F4 d{(h.x - c.x) + l.x, (h.y - c.y) + l.y, (h.z - c.z) + l.z, h.w};
+      s_hi[q] = h;
+      s_lo[q] = l;
+      s_rel[q] = d;
+      r2max = fmaxf(r2max, __builtin_fmaf(d.x, d.x, __builtin_fmaf(d.y, d.y, d.z * d.z)));
+    }
+    const float dx = (pi.x - c.x) + li.x, dy = (pi.y - c.y) + li.y, dz = (pi.z - c.z) + li.z;
+    float d2min = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    for (int o = 32; o > 0; o >>= 1) {
+      r2max = fmaxf(r2max, __shfl_xor(r2max, o));
+      d2min = fminf(d2min, __shfl_xor(d2min, o));
+    }
+    if (lane == 0) {
+      s_red[2 * wave] = r2max;
+      s_red[2 * wave + 1] = d2min;
+    }
+    __syncthreads();
+    float R2 = s_red[0], D2 = s_red[1];
+    for (int w = 1; w < kGravBlock / 64; ++w) {
+      R2 = fmaxf(R2, s_red[2 * w]);
+      D2 = fminf(D2, s_red[2 * w + 1]);
+    }
+    // |d_i| >= 3 R_t for every lane; the squared test has a margin of 1.1
+    // for the rounding of the squares
+    const bool tile_form = !split_only && D2 >= 9.9f * R2;
+    auto pair = [&](float rx, float ry, float rz, float mj) {
+      const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+      const float inv = __builtin_amdgcn_rsqf(d2);
+      const float mi = mj * inv;
+      const float s3 = mi * (inv * inv);
+      a[0].fsub(rx, s3);
+      a[1].fsub(ry, s3);
+      a[2].fsub(rz, s3);
+      a[3].sub(mi);
+    };
+    auto pair_tile = [&](int j) {
+      const F4 pj = s_rel[j];
+      pair(dx - pj.x, dy - pj.y, dz - pj.z, pj.w);
+    };
+    auto pair_split = [&](int j) {
+      const F4 pj = s_hi[j], lj = s_lo[j];
+      pair((pi.x - pj.x) + (li.x - lj.x), (pi.y - pj.y) + (li.y - lj.y), (pi.z - pj.z) + (li.z - lj.z), pj.w);
+    };
     for (int j0 = 0; j0 < cnt; j0 += kTile) {
-      const int j1 = min(j0 + kTile, cnt);
-      for (int j = j0; j < j1; ++j) {
-        const F4 pj = s_hi[j], lj = s_lo[j];
-        const float rx = (pi.x - pj.x) + (li.x - lj.x);
-        const float ry = (pi.y - pj.y) + (li.y - lj.y);
-        const float rz = (pi.z - pj.z) + (li.z - lj.z);
-        const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-        const float inv = __builtin_amdgcn_rsqf(d2);
-        const float mi = pj.w * inv;
-        const float s3 = mi * (inv * inv);
-        a[0].fsub(rx, s3);
-        a[1].fsub(ry, s3);
-        a[2].fsub(rz, s3);
-        a[3].sub(mi);
+      // full blocks (every block when n_j is a multiple of kTile) have a
+      // fixed trip count, unrolled so LDS reads are issued ahead of their use
+      if (j0 + kTile <= cnt) {
+        if (tile_form) {
+#pragma unroll 16
+          for (int j = 0; j < kTile; ++j) pair_tile(j0 + j);
+        } else {
+#pragma unroll 16
+          for (int j = 0; j < kTile; ++j) pair_split(j0 + j);
+        }
+      } else {
+        for (int j = j0; j < cnt; ++j) {
+          if (tile_form) pair_tile(j);
+          else pair_split(j);
+        }
       }
       for (int k = 0; k < 4; ++k) a[k].tile_end();
     }
@@ -285,10 +350,10 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
 }
 
 void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, int n_i, int n_j, int soft,
-                          float eps, hipStream_t s) {
+                          float eps, int split_only, hipStream_t s) {
   const dim3 grid(ceil_div(n_i, kGravBlock)), block(kGravBlock);
   if (soft == 0) hipLaunchKernelGGL(k_gravity_mixed_spline, grid, block, 0, s, hi, lo, gvel, out, n_i, n_j);
-  else hipLaunchKernelGGL(k_gravity_mixed_plummer, grid, block, 0, s, hi, lo, out, n_i, n_j, eps * eps);
+  else hipLaunchKernelGGL(k_gravity_mixed_plummer, grid, block, 0, s, hi, lo, out, n_i, n_j, eps * eps, split_only);
   CS_CHECK_LAUNCH();
 }
 

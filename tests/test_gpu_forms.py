@@ -110,14 +110,15 @@ def test_mixed_form_trajectory_against_fp64(cs, orc, cfg):
     assert drift <= 1e-6, drift
 
 
-@pytest.mark.parametrize("cfg", [2, 3])
-def test_mixed_form_passes_against_fp64_oracle(cs, orc, cfg):
+@pytest.mark.parametrize("cfg,ph", [(2, False), (3, False), (2, True), (3, True)])
+def test_mixed_form_passes_against_fp64_oracle(cs, orc, cfg, ph):
     """Density, EOS, hydro + reaction and gravity of the mixed form against the
     FP64 oracle run on the mixed form's own state (same ranges): rho and P to
-    2e-6, accelerations to 1e-5 of max(|a|, 1e-3 rms)."""
-    print(f"mixed passes cfg={cfg}", flush=True)
+    2e-6, accelerations to 1e-5 of max(|a|, 1e-3 rms).  In Peano-Hilbert order
+    the gravity kernel takes its tile-relative form for distant tiles."""
+    print(f"mixed passes cfg={cfg} ph_sort={ph}", flush=True)
     n = 16384
-    e = cs.make_engine("mixed", cfg, n, 41)
+    e = cs.make_engine("mixed", cfg, n, 41, ph_sort=ph)
     e.init()
     e.step()
     for pid in (cs.SPH_ACC, cs.GRAVITY):
@@ -155,6 +156,25 @@ def test_mixed_form_passes_against_fp64_oracle(cs, orc, cfg):
         floor = 1e-3 * np.sqrt((b**2).sum(1).mean())
         err = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), floor)
         assert err.max() < 1e-5, (name, err.max())
+
+
+def test_mixed_gravity_forms_agree(cs):
+    """The tile-relative and split forms of the mixed gravity kernel give the
+    same accelerations to FP32 rounding level on Peano-Hilbert-ordered input."""
+    n = 65536
+    out = {}
+    for split in (0, 1):
+        e = cs.make_engine("mixed", 3, n, 7, ph_sort=True)
+        e.set_scalar("grav_split_only", split)
+        e.run_pass(cs.GRAVITY)
+        out[split] = e.get("grav")[:n].astype(np.float64)
+        e.close()
+    a, b = out[0][:, :3], out[1][:, :3]
+    rms = np.sqrt((b ** 2).sum(1).mean())
+    err = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-3 * rms)
+    assert err.max() < 2e-6, err.max()
+    assert not np.array_equal(a, b), "the tile-relative form was not taken"
+    np.testing.assert_allclose(out[0][:, 3], out[1][:, 3], rtol=2e-6)
 
 
 def test_state_file_round_trip(cs, tmp_path):
