@@ -55,10 +55,14 @@ def run_form(cs, form, n, seed, warmup, steps, timing=True, i_order=0):
     e.sync()
     wall = time.perf_counter() - t0
     timers = e.timers()
-    mom = e.stats()
+    st = e.stats()
     kbar = float(e.get("ngb_count")[:n].mean())
+    import numpy as np
+    v = (e.get("vel64") if form == "mixed" else e.get("vel"))[:n].astype(np.float64)
+    mom = (v[:, :3] * v[:, 3:4]).sum(0)
     out = {"wall_s": wall, "kbar": kbar, "energies": energies, "timers": timers, "dt": e.scalar("dt"),
-           "momentum_residual": [mom["Fx"], mom["Fy"], mom["Fz"]]}
+           "momentum": mom, "momentum_scale": float((np.linalg.norm(v[:, :3], axis=1) * v[:, 3]).sum()),
+           "angular_momentum": st["J"]}
     e.close()
     return out
 
@@ -249,6 +253,13 @@ def main():
         return {"max_per_step": max(per[1:]) if len(per) > 1 else 0.0,
                 "final": abs(run["energies"][-1] - fp64["energies"][-1]) / e0}
 
+    def mom_drift(run):
+        if run is None or fp64 is None:
+            return None
+        import numpy as np
+        return {"linear": float(np.linalg.norm(run["momentum"] - fp64["momentum"]) / fp64["momentum_scale"]),
+                "angular": abs(run["angular_momentum"] - fp64["angular_momentum"]) / abs(fp64["angular_momentum"])}
+
     line = {
         "metric": "particle-updates/s @1M (mixed vs plain FP32, rel err <=1e-6); adjoint/fwd ratio",
         "value": value,
@@ -270,6 +281,9 @@ def main():
         "mixed_over_plain": None if plain is None else plain["wall_s"] / mixed["wall_s"],
         "energy_drift_vs_fp64": {"definition": "|E_form(k) - E_fp64(k)| / |E_fp64(0)| / k over the run",
                                  "mixed": drift(mixed), "plain": drift(plain)},
+        "momentum_drift_vs_fp64": {
+            "definition": "|P_form - P_fp64| / sum(m |v|) and |J_form - J_fp64| / |J_fp64| after the run",
+            "mixed": mom_drift(mixed), "plain": mom_drift(plain)},
         "kernel_ms_per_step": {k: v[0] / max(K, 1) for k, v in sorted(mixed["timers"].items())},
         "sph_hbm": {"mean_list_length": mixed["kbar"],
                     "passes": sph_bandwidth(mixed["timers"], K, n, mixed["kbar"])},
