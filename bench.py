@@ -196,6 +196,53 @@ def adjoint_leg(cs, n, steps, ckpt, fd_dirs, seed):
     }
 
 
+SCALE_WORKLOAD = "uniform-density sphere (R=1, M=1) with a Gaussian k^-4 velocity field from a 128^3 grid, " \
+                 "rms Mach 2; u=1, gas+radiation EOS; 64+-8 neighbours; Plummer eps=0.001"
+
+
+def scale_leg(cs, n, steps, seed):
+    """The scale workload: `steps` mixed-form steps (SPH GB/s, gravity FP32
+    fraction) and one reverse-mode step of the FP64 form at the same N."""
+    e = cs.make_engine("mixed", 5, n, seed, ph_sort=True, i_order=0)
+    e.set_scalar("timing", 1)
+    e.init()
+    e.reset_timers()
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.sync()
+    wall = time.perf_counter() - t0
+    timers = e.timers()
+    kbar = float(e.get("ngb_count")[:n].mean())
+    nd = e.nd
+    e.close()
+    g_ms, g_calls = timers.get("gravity", (float("nan"), 1))
+    g_avg = g_ms / max(g_calls, 1) / 1e3
+    achieved = FLOP_PER_PAIR * float(nd) * float(nd) / g_avg / 1e12
+    out = {"workload": SCALE_WORKLOAD, "n": n, "steps": steps, "form": "mixed",
+           "value": n * steps / wall, "unit": "particle-updates/s", "ms_per_step": wall / steps * 1e3,
+           "kernel_ms_per_step": {k: v[0] / steps for k, v in sorted(timers.items())},
+           "sph_hbm": {"mean_list_length": kbar, "passes": sph_bandwidth(timers, steps, n, kbar)},
+           "gravity": {"kernel": "k_gravity_mixed_plummer", "launch_ms": g_avg * 1e3, "achieved": achieved,
+                       "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS}}
+    print(json.dumps({"scale_progress": "mixed steps done", "ms_per_step": out["ms_per_step"]}), flush=True)
+    p = cs.preset_params(5, n)
+    pos, vel, hyd = cs.generate_ic(5, n, seed)
+    f = cs.Engine("fp64", p)
+    f.load_ic(pos, vel, hyd)
+    f.sync()
+    t0 = time.perf_counter()
+    loss = f.adjoint_forward(1, 1)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    f.adjoint_backward()
+    t_bwd = time.perf_counter() - t0
+    f.close()
+    out["adjoint_step"] = {"form": "fp64", "steps": 1, "loss": loss, "forward_s": t_fwd, "backward_s": t_bwd,
+                           "adjoint_over_forward": t_bwd / t_fwd}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +262,10 @@ def main():
     ap.add_argument("--fd-dirs", type=int, default=8,
                     help="seeded unit directions for the finite-difference check of the gradient")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the reverse-mode leg")
+    ap.add_argument("--scale", action="store_true",
+                    help="run only the scale workload (N = 4,194,304: 5 mixed steps + one adjoint step)")
+    ap.add_argument("--scale-n", type=int, default=4194304)
+    ap.add_argument("--scale-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (for scaling memory counters)")
     args = ap.parse_args()
@@ -231,6 +282,9 @@ def main():
 
     import cudasph_amd as cs
     n, K, W = args.n, args.steps, args.warmup
+    if args.scale:
+        print(json.dumps(scale_leg(cs, args.scale_n, args.scale_steps, args.seed)))
+        return
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(cs, args.adjoint_n, args.adjoint_steps, args.adjoint_ckpt, args.fd_dirs,
                                      args.seed)))

@@ -117,6 +117,7 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
     ghi_.alloc(nd_);
     glo_.alloc(nd_);
     gvel32_.alloc(nd_);
+    gpart_.alloc(2 * 8 * (size_t)nd_);
     CS_HIP(hipMemsetAsync(pos_lo_, 0, pos_lo_.bytes(), s_));
     CS_HIP(hipMemsetAsync(pos64_, 0, pos64_.bytes(), s_));
     CS_HIP(hipMemsetAsync(vel64_, 0, vel64_.bytes(), s_));
@@ -487,7 +488,7 @@ void Engine<T>::gravity_compute() {
     if (mixed()) {
       launch_gravity_pack_mixed(pos64_, pos_, vel_, ghi_, glo_, gvel32_, n_, nd_, s_);
       TimerScope ts(timers_, "gravity", s_);
-      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, nd_, nd_, p_.grav_mode, p_.grav_eps, grav_split_only_,
+      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, nd_, nd_, p_.grav_mode, p_.grav_eps, grav_split_only_,
                            s_);
       return;
     }

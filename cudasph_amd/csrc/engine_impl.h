@@ -153,7 +153,7 @@ class Engine final : public EngineBase {
   // mixed form
   DBuf<D4> pos64_, vel64_, old64_;
   DBuf<double> u64_;
-  DBuf<F4> pos_lo_, ghi_, glo_, gvel32_;
+  DBuf<F4> pos_lo_, ghi_, glo_, gvel32_, gpart_;
   bool dirty64_ = false;
 
   HBuf<V4<T>> h_pos_, h_tree_;

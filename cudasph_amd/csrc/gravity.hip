@@ -73,8 +73,13 @@ __device__ __forceinline__ float inv_sqrt(float x) {
   else return __builtin_amdgcn_rsqf(x);
 }
 template <bool EXACT>
+// FP64: hardware reciprocal square root refined by two Newton steps (each
+// squares the relative error; the result is within a few ulp of 1/sqrt(x)),
+// in place of the long correctly rounded sqrt + divide sequences
 __device__ __forceinline__ double inv_sqrt(double x) {
-  return 1.0 / __builtin_sqrt(x);
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * (1.5 - 0.5 * x * y * y);
+  return y * (1.5 - 0.5 * x * y * y);
 }
 
 // spline-softened pair (sph_grav_kernel.cu:14-111); pj = (x,y,z,m),
@@ -147,6 +152,7 @@ __device__ __forceinline__ void pair_plummer(A* a, T rx, T ry, T rz, T mj, T eps
 constexpr int kGravBlock = 256;
 constexpr int kTile = 64;        // j per blocked-compensation partial (mixed form)
 constexpr int kJTile = 512;      // j records staged in LDS per block iteration
+constexpr int kGravParts = 8;    // j-parts of the mixed Plummer kernel, one per XCD
 
 // Block-cooperative staging of j records into LDS.  Every lane of a wave then
 // reads the same LDS address (a broadcast read), so one j record serves the
@@ -180,11 +186,20 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
       for (int q = threadIdx.x; q < cnt; q += kGravBlock) s_r2[q] = gvel[t0 + q].w;
       __syncthreads();
     }
-    for (int j = 0; j < cnt; ++j) {
+    auto body = [&](int j) {
       const V4<T> pj = s_pos[j];
       const T rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
       if constexpr (SOFT == 0) pair_spline<T, EXACT>(a, rx, ry, rz, ri2, pj.w, s_r2[j]);
       else pair_plummer<T, EXACT>(a, rx, ry, rz, pj.w, eps2);
+    };
+    // j stays in ascending order (the reference's summation order); full
+    // tiles have a fixed trip count so the loop unrolls and LDS reads are
+    // issued ahead of their use
+    if (cnt == kJTile) {
+#pragma unroll 8
+      for (int j = 0; j < kJTile; ++j) body(j);
+    } else {
+      for (int j = 0; j < cnt; ++j) body(j);
     }
   }
   if (i < n_i) out[i] = {a[0].get(), a[1].get(), a[2].get(), a[3].get()};
@@ -211,19 +226,31 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
 // the split form.  split_only = 1 forces the split form everywhere.
 // After r: 3 FMAs for r^2 + eps^2, one rsq, 3 multiplies, 3 FMAs into the
 // partial acceleration, 1 subtraction for the potential.
+//
+// Grid: kGravParts j-parts per i-block, block b = i_block * kGravParts + part.
+// Workgroups are dispatched round-robin over the 8 XCDs, so part p runs on
+// XCD p and each XCD streams only its 1/8 of the j-set (4 MiB of hi+lo
+// records at N = 2^20), which its L2 holds; without the split every XCD
+// streams the whole j-set and re-fetches it from HBM.  Each block writes its
+// compensated (hi, lo) partial sums; k_gravity_combine adds the parts in
+// order.
 __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* __restrict__ hi,
                                                                        const F4* __restrict__ lo,
-                                                                       F4* __restrict__ out, int n_i, int n_j,
-                                                                       float eps2, int split_only) {
+                                                                       F4* __restrict__ part_hi,
+                                                                       F4* __restrict__ part_lo, int n_i,
+                                                                       int n_j, int j_chunk, float eps2,
+                                                                       int split_only) {
   __shared__ F4 s_hi[kJTile], s_lo[kJTile], s_rel[kJTile];
   __shared__ float s_red[2 * (kGravBlock / 64)];
-  const int i = blockIdx.x * kGravBlock + threadIdx.x;
+  const int part = blockIdx.x % kGravParts;
+  const int i = (blockIdx.x / kGravParts) * kGravBlock + threadIdx.x;
   const int ii = i < n_i ? i : n_i - 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const F4 pi = hi[ii], li = lo[ii];
   GAcc<float, 1> a[4];
-  for (int t0 = 0; t0 < n_j; t0 += kJTile) {
-    const int cnt = min(kJTile, n_j - t0);
+  const int j_begin = min(part * j_chunk, n_j), j_end = min(j_begin + j_chunk, n_j);
+  for (int t0 = j_begin; t0 < j_end; t0 += kJTile) {
+    const int cnt = min(kJTile, j_end - t0);
     const F4 c = hi[t0 + cnt / 2];   // the middle record: about half the tile's extent from either end
     __syncthreads();
     float r2max = 0.f;
@@ -292,7 +319,32 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
       for (int k = 0; k < 4; ++k) a[k].tile_end();
     }
   }
-  if (i < n_i) out[i] = {a[0].get(), a[1].get(), a[2].get(), a[3].get()};
+  if (i < n_i) {
+    const size_t o = (size_t)part * n_i + i;
+    part_hi[o] = {a[0].v.hi, a[1].v.hi, a[2].v.hi, a[3].v.hi};
+    part_lo[o] = {a[0].v.lo, a[1].v.lo, a[2].v.lo, a[3].v.lo};
+  }
+}
+
+// sum of the kGravParts compensated partials of each i, in part order
+__global__ __launch_bounds__(kBlock) void k_gravity_combine(const F4* __restrict__ part_hi,
+                                                            const F4* __restrict__ part_lo, F4* __restrict__ out,
+                                                            int n_i) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n_i) return;
+  DR a[4] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+  for (int p = 0; p < kGravParts; ++p) {
+    const F4 h = part_hi[(size_t)p * n_i + i], l = part_lo[(size_t)p * n_i + i];
+    dr_add(a[0], h.x);
+    dr_add(a[0], l.x);
+    dr_add(a[1], h.y);
+    dr_add(a[1], l.y);
+    dr_add(a[2], h.z);
+    dr_add(a[2], l.z);
+    dr_add(a[3], h.w);
+    dr_add(a[3], l.w);
+  }
+  out[i] = {a[0].hi + a[0].lo, a[1].hi + a[1].lo, a[2].hi + a[2].lo, a[3].hi + a[3].lo};
 }
 
 // mixed form, spline softening: same accumulation and split positions
@@ -349,11 +401,21 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
   CS_CHECK_LAUNCH();
 }
 
-void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, int n_i, int n_j, int soft,
-                          float eps, int split_only, hipStream_t s) {
+void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, int n_i, int n_j,
+                          int soft, float eps, int split_only, hipStream_t s) {
   const dim3 grid(ceil_div(n_i, kGravBlock)), block(kGravBlock);
-  if (soft == 0) hipLaunchKernelGGL(k_gravity_mixed_spline, grid, block, 0, s, hi, lo, gvel, out, n_i, n_j);
-  else hipLaunchKernelGGL(k_gravity_mixed_plummer, grid, block, 0, s, hi, lo, out, n_i, n_j, eps * eps, split_only);
+  if (soft == 0) {
+    hipLaunchKernelGGL(k_gravity_mixed_spline, grid, block, 0, s, hi, lo, gvel, out, n_i, n_j);
+  } else {
+    const int chunk = ceil_div(ceil_div(n_j, kGravParts), kTile) * kTile;
+    F4* part_hi = part;
+    F4* part_lo = part + (size_t)kGravParts * n_i;
+    hipLaunchKernelGGL(k_gravity_mixed_plummer, dim3(ceil_div(n_i, kGravBlock) * kGravParts), block, 0, s, hi, lo,
+                       part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only);
+    CS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_gravity_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part_hi, part_lo, out,
+                       n_i);
+  }
   CS_CHECK_LAUNCH();
 }
 
