@@ -208,22 +208,22 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
 // mixed form, Plummer: split positions, FMA pair arithmetic, blocked
 // compensated sums.  hi = (x,y,z,m) rounded, lo = (x-hi, y-hi, z-hi, 0).
 //
-// Two ways to form r = x_i - x_j, chosen per (block, j-tile):
+// Two ways to form r = x_i - x_j, chosen per (wavefront, j-tile):
 //  split:  r = (hi_i - hi_j) + (lo_i - lo_j), 9 operations per pair, relative
 //          error ~2^-24 for every pair;
 //  tile:   the tile is expressed relative to its middle record c,
 //          d_j = (hi_j - c) + lo_j (staged in LDS) and d_i = (hi_i - c) + lo_i
 //          (once per tile per lane), r = d_i - d_j: 3 operations per pair,
 //          absolute error <= 2^-24 (|d_i| + |d_j|) + rounding of r.
-// The tile form is used only when every lane of the block has
+// The tile form is used only when every lane of the wavefront has
 // |d_i| >= 3 R_t, R_t = max_j |d_j|: then |r| >= |d_i| - R_t >= 2 R_t and
 // (|d_i| + |d_j|) / |r| <= (|d_i| + R_t) / (|d_i| - R_t) <= 2, so r keeps a
-// relative error of a few 2^-24 -- the accuracy of the split form.  Blocks
-// near a tile (the pairs whose separation the split coordinates exist for)
-// use the split form.  With particles in Peano-Hilbert order the tiles are
-// compact and most (block, tile) combinations are far apart; with an
-// unordered input the tiles are as large as the system and every block uses
-// the split form.  split_only = 1 forces the split form everywhere.
+// relative error of a few 2^-24 -- the accuracy of the split form.
+// Wavefronts near a tile (the pairs whose separation the split coordinates
+// exist for) use the split form.  With particles in Peano-Hilbert order the
+// tiles are compact and most (wavefront, tile) combinations are far apart;
+// with an unordered input the tiles are as large as the system and every
+// wavefront uses the split form.  split_only = 1 forces the split form everywhere.
 // After r: 3 FMAs for r^2 + eps^2, one rsq, 3 multiplies, 3 FMAs into the
 // partial acceleration, 1 subtraction for the potential.
 //
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
                                                                        int n_j, int j_chunk, float eps2,
                                                                        int split_only) {
   __shared__ F4 s_hi[kJTile], s_lo[kJTile], s_rel[kJTile];
-  __shared__ float s_red[2 * (kGravBlock / 64)];
+  __shared__ float s_red[kGravBlock / 64];
   const int part = blockIdx.x % kGravParts;
   const int i = (blockIdx.x / kGravParts) * kGravBlock + threadIdx.x;
   const int ii = i < n_i ? i : n_i - 1;
@@ -268,19 +268,14 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
       r2max = fmaxf(r2max, __shfl_xor(r2max, o));
       d2min = fminf(d2min, __shfl_xor(d2min, o));
     }
-    if (lane == 0) {
-      s_red[2 * wave] = r2max;
-      s_red[2 * wave + 1] = d2min;
-    }
+    if (lane == 0) s_red[wave] = r2max;
     __syncthreads();
-    float R2 = s_red[0], D2 = s_red[1];
-    for (int w = 1; w < kGravBlock / 64; ++w) {
-      R2 = fmaxf(R2, s_red[2 * w]);
-      D2 = fminf(D2, s_red[2 * w + 1]);
-    }
-    // |d_i| >= 3 R_t for every lane; the squared test has a margin of 1.1
+    float R2 = s_red[0];
+    for (int w = 1; w < kGravBlock / 64; ++w) R2 = fmaxf(R2, s_red[w]);
+    // |d_i| >= 3 R_t for every lane of this wave (the choice is uniform per
+    // wave, so there is no divergence); the squared test has a margin of 1.1
     // for the rounding of the squares
-    const bool tile_form = !split_only && D2 >= 9.9f * R2;
+    const bool tile_form = !split_only && d2min >= 9.9f * R2;
     auto pair = [&](float rx, float ry, float rz, float mj) {
       const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
       const float inv = __builtin_amdgcn_rsqf(d2);

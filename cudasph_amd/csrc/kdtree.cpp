@@ -34,7 +34,7 @@ inline int left_size(int n) {
   return last <= m / 2 ? (m - 2) / 2 + last : (m - 2) / 2 + m / 2;
 }
 
-constexpr int kParallelGrain = 32768;   // elements below which work stays on the calling thread
+constexpr int kParallelGrain = 4096;   // elements below which work stays on the calling thread
 
 template <class T>
 struct Builder {

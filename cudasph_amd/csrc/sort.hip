@@ -204,10 +204,13 @@ __global__ __launch_bounds__(kBlock) void k_reaction_pairs(const int* __restrict
                                                             const int* __restrict__ count,
                                                             const int* __restrict__ list, KV* __restrict__ pairs,
                                                             int* __restrict__ owner, int n) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  // one wavefront per particle: its lanes write the particle's slot run, so
+  // the stores are contiguous across the wavefront
+  const int i = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (i >= n) return;
   const int off = offset[i], end = offset[i + 1], c = count[i];
-  for (int k = off; k < end; ++k) {
+  for (int k = off + lane; k < end; k += 64) {
     pairs[k] = KV{k - off < c ? (uint32_t)list[k] : (uint32_t)n, (uint32_t)k};
     owner[k] = i;
   }
@@ -215,8 +218,8 @@ __global__ __launch_bounds__(kBlock) void k_reaction_pairs(const int* __restrict
 
 void launch_reaction_pairs(const int* offset, const int* count, const int* list, KV* pairs, int* owner, int n,
                            hipStream_t s) {
-  hipLaunchKernelGGL(k_reaction_pairs, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, offset, count, list, pairs,
-                     owner, n);
+  hipLaunchKernelGGL(k_reaction_pairs, dim3(ceil_div(n, kBlock / 64)), dim3(kBlock), 0, s, offset, count, list,
+                     pairs, owner, n);
   CS_CHECK_LAUNCH();
 }
 
