@@ -14,9 +14,10 @@
 // timesteps and gravity of every force evaluation recorded, and the state
 // checkpointed every k steps.
 // Backward (adjoint_backward): segments of k steps in reverse; each segment is
-// recomputed from its checkpoint with the recorded ranges, timesteps and
-// gravity (no N^2 pass), storing the pre-step states, and then every step is
-// re-evaluated once more (predict + SPH passes) and reversed:
+// recomputed once from its checkpoint with the recorded ranges, timesteps and
+// gravity (no N^2 pass), capturing every step's force-evaluation data (the
+// state at the evaluation, the neighbour lists and their sorted transpose),
+// and then its steps are reversed from last to first:
 //   correct:  a'_bar += v'_bar f, udot'_bar += u'_bar f, f = dt_new/2 + dt_old/2
 //   forces:   hydro pairs -> transpose gather -> EOS -> density pairs ->
 //             transpose gather -> Plummer gravity (one FP64 N^2 pass)
@@ -582,6 +583,51 @@ void Engine<T>::adj_load(DBuf<V4<T>>* src[4], long slot) {
 }
 
 template <class T>
+void Engine<T>::adj_capture(EvalSlot& e) {
+  const size_t nd = (size_t)nd_, tot = (size_t)ngb_tot_;
+  auto grow = [](auto& b, size_t n) {
+    if (b.n < n) b.alloc(n);
+  };
+  grow(e.pos, nd);
+  grow(e.vel, nd);
+  grow(e.hyd, nd);
+  grow(e.dots, nd);
+  grow(e.off, nd + 1);
+  grow(e.cnt, nd);
+  grow(e.tr_start, nd);
+  grow(e.tr_end, nd);
+  grow(e.list, std::max<size_t>(tot, 1));
+  grow(e.pairs, std::max<size_t>(tot, 1));
+  d2d(e.pos.p, pos_.p, nd, s_);
+  d2d(e.vel.p, vel_.p, nd, s_);
+  d2d(e.hyd.p, hydro_.p, nd, s_);
+  d2d(e.dots.p, dots_.p, nd, s_);
+  d2d(e.off.p, ngb_offset_.p, nd + 1, s_);
+  d2d(e.cnt.p, ngb_count_.p, nd, s_);
+  d2d(e.tr_start.p, tr_start_.p, nd, s_);
+  d2d(e.tr_end.p, tr_end_.p, nd, s_);
+  d2d(e.list.p, list_.p, tot, s_);
+  d2d(e.pairs.p, pairs_.p, tot, s_);
+  e.tot = ngb_tot_;
+}
+
+template <class T>
+void Engine<T>::adj_restore(const EvalSlot& e) {
+  const size_t nd = (size_t)nd_, tot = (size_t)e.tot;
+  d2d(pos_.p, e.pos.p, nd, s_);
+  d2d(vel_.p, e.vel.p, nd, s_);
+  d2d(hydro_.p, e.hyd.p, nd, s_);
+  d2d(dots_.p, e.dots.p, nd, s_);
+  d2d(ngb_offset_.p, e.off.p, nd + 1, s_);
+  d2d(ngb_count_.p, e.cnt.p, nd, s_);
+  d2d(tr_start_.p, e.tr_start.p, nd, s_);
+  d2d(tr_end_.p, e.tr_end.p, nd, s_);
+  d2d(list_.p, e.list.p, tot, s_);
+  d2d(pairs_.p, e.pairs.p, tot, s_);
+  ngb_tot_ = e.tot;
+}
+
+template <class T>
 double Engine<T>::adj_loss() {
   if constexpr (kF32) {
     throw std::runtime_error("adjoint: FP64 form only");
@@ -636,7 +682,11 @@ namespace {
 // restores the engine's normal mode when an adjoint call ends, also on error
 struct ModeGuard {
   int& mode;
-  ~ModeGuard() { mode = 0; }
+  int& capture;
+  ~ModeGuard() {
+    mode = 0;
+    capture = -1;
+  }
 };
 }  // namespace
 
@@ -654,7 +704,8 @@ double Engine<T>::adjoint_forward(int steps, int ckpt_every) {
     adj_nngb_.alloc((K + 1) * nd);
     adj_grav_.alloc((K + 1) * nd);
     for (auto* b : {&adj_ck_pos_, &adj_ck_vel_, &adj_ck_hyd_, &adj_ck_bd_}) b->alloc(std::max(nseg, 1) * nd);
-    for (auto* b : {&adj_seg_pos_, &adj_seg_vel_, &adj_seg_hyd_, &adj_seg_bd_}) b->alloc(ck * nd);
+    adj_eval_.clear();
+    for (int c = 0; c < ck; ++c) adj_eval_.push_back(std::make_unique<EvalSlot>());
     for (auto* b : {&adj_init_pos_, &adj_init_vel_, &adj_init_hyd_, &adj_final_vel_}) b->alloc(nd);
     for (auto* b : {&ax_, &av_, &aa_, &xs_, &vs_, &hb1_, &hb2_}) b->alloc(nd);
     au_.alloc(nd);
@@ -665,7 +716,7 @@ double Engine<T>::adjoint_forward(int steps, int ckpt_every) {
     adj_K_ = K;
     adj_k_ = ck;
     adj_iter0_ = (int)iteration_;
-    ModeGuard g{adj_mode_};
+    ModeGuard g{adj_mode_, adj_capture_};
     adj_mode_ = kAdjRecord;
     init();
     DBuf<V4<T>>* ckb[4] = {&adj_ck_pos_, &adj_ck_vel_, &adj_ck_hyd_, &adj_ck_bd_};
@@ -688,7 +739,7 @@ void Engine<T>::adjoint_backward(double* gx, double* gv, double* gu) {
     if (!adj_ready_) throw std::runtime_error("adjoint_backward: run adjoint_forward first");
     const int K = adj_K_, ck = adj_k_, nseg = (K + ck - 1) / ck;
     const size_t nd = (size_t)nd_;
-    ModeGuard g{adj_mode_};
+    ModeGuard g{adj_mode_, adj_capture_};
     adj_mode_ = kAdjReplayStoredGravity;
     for (auto* b : {&ax_, &aa_}) CS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), s_));
     CS_HIP(hipMemsetAsync(au_.p, 0, au_.bytes(), s_));
@@ -696,27 +747,21 @@ void Engine<T>::adjoint_backward(double* gx, double* gv, double* gu) {
     CS_CHECK_LAUNCH();
     if (nd_ > n_) CS_HIP(hipMemsetAsync(av_.p + n_, 0, (nd - n_) * sizeof(D4v), s_));
     DBuf<V4<T>>* ckb[4] = {&adj_ck_pos_, &adj_ck_vel_, &adj_ck_hyd_, &adj_ck_bd_};
-    DBuf<V4<T>>* seg[4] = {&adj_seg_pos_, &adj_seg_vel_, &adj_seg_hyd_, &adj_seg_bd_};
-    auto set_dt = [&](int f) { CS_HIP(hipMemcpyAsync(dt_cur_, &adj_dt_[f], 8, hipMemcpyHostToDevice, s_)); };
     for (int sg = nseg - 1; sg >= 0; --sg) {
       const int first = sg * ck + 1, last = std::min((sg + 1) * ck, K);
-      // recompute the segment's pre-step states from its checkpoint
+      // recompute the segment from its checkpoint, capturing each step's
+      // force-evaluation data (the state at the evaluation, lists, transpose)
       adj_load(ckb, sg);
-      set_dt(first - 1);
+      CS_HIP(hipMemcpyAsync(dt_cur_, &adj_dt_[first - 1], 8, hipMemcpyHostToDevice, s_));
       iteration_ = adj_iter0_ + first - 1;
       for (int s = first; s <= last; ++s) {
-        adj_save(seg, s - first);
-        if (s < last) step();
+        adj_capture_ = s - first;
+        step();
       }
-      // reverse the segment, re-evaluating each step's forces first
+      adj_capture_ = -1;
+      // reverse the segment
       for (int s = last; s >= first; --s) {
-        adj_load(seg, s - first);
-        set_dt(s - 1);
-        iteration_ = adj_iter0_ + s;
-        adj_f_ = s;
-        predict();
-        tree();
-        sph_after_tree();
+        adj_restore(*adj_eval_[s - first]);
         const double f = 0.5 * adj_dt_[s] + 0.5 * adj_dt_[s - 1];
         hipLaunchKernelGGL(k_adj_correct, grid(nd_), dim3(kBlock), 0, s_, aa_.p, av_.p, au_.p, f, n_);
         CS_CHECK_LAUNCH();
@@ -770,7 +815,7 @@ double Engine<T>::adjoint_replay(const double* x0, const double* v0, const doubl
       CS_CHECK_LAUNCH();
     }
     sync();   // the host arrays may be freed once this returns
-    ModeGuard g{adj_mode_};
+    ModeGuard g{adj_mode_, adj_capture_};
     adj_mode_ = kAdjReplay;
     iteration_ = adj_iter0_;
     init();
@@ -783,6 +828,8 @@ double Engine<T>::adjoint_replay(const double* x0, const double* v0, const doubl
   template void Engine<T>::adj_save(DBuf<V4<T>>* [4], long);                    \
   template void Engine<T>::adj_load(DBuf<V4<T>>* [4], long);                    \
   template double Engine<T>::adj_loss();                                        \
+  template void Engine<T>::adj_capture(EvalSlot&);                              \
+  template void Engine<T>::adj_restore(const EvalSlot&);                        \
   template void Engine<T>::adj_forces_backward(bool);                           \
   template double Engine<T>::adjoint_forward(int, int);                         \
   template void Engine<T>::adjoint_backward(double*, double*, double*);         \

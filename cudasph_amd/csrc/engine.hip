@@ -80,7 +80,10 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
   if (form == SPH_FORM_MIXED && !kF32) throw std::invalid_argument("sph_engine: mixed form is FP32");
   CS_HIP(hipSetDevice(device));
   CS_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+  CS_HIP(hipStreamCreateWithFlags(&gs_, hipStreamNonBlocking));   // default priority, as s_
   CS_HIP(hipEventCreateWithFlags(&fetched_, hipEventDisableTiming));
+  CS_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+  CS_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
   cap_ = 100L * nd_;  // ngb_tot_max (read_state.cpp:29)
 
   for (auto* b : {&pos_, &vel_, &hydro_, &dots_, &bdots_, &hydro2_, &grav_, &gpos_, &gvel_, &old_vu_, &tmp4_})
@@ -231,6 +234,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "exact_rsqrt") return exact_rsqrt_;
   if (name == "i_order") return i_order_;
   if (name == "grav_split_only") return grav_split_only_;
+  if (name == "overlap") return overlap_;
   if (name == "timing") return timers_.enabled;
   if (name == "cap") return (double)cap_;
   if (name == "real_bytes") return (double)sizeof(T);
@@ -254,6 +258,8 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     i_order_ = (int)v;
   } else if (name == "grav_split_only") {
     grav_split_only_ = (int)v;
+  } else if (name == "overlap") {
+    overlap_ = (int)v;
   } else if (name == "timing") {
     sync();
     timers_.enabled = v != 0;
@@ -468,34 +474,64 @@ void Engine<T>::sph_after_tree() {
 }
 
 // CUDA_gravity.cpp:4-71 + sph_grav_kernel.cu
+// Gravity.  async: the pack runs on the main stream (it reads the predicted
+// positions before the h-solve rewrites their range words), the kernels on the
+// gravity stream gs_, so the SPH passes queued next on the main stream run
+// beside them; gravity_join() makes the main stream wait for the result.
 template <class T>
-void Engine<T>::gravity() {
+void Engine<T>::gravity(bool async) {
   refresh_if_dirty();
   if (adj_mode_ == kAdjReplayStoredGravity) {
     CS_HIP(hipMemcpyAsync(grav_, adj_grav_.p + (size_t)adj_f_ * nd_, (size_t)nd_ * sizeof(V4<T>),
                           hipMemcpyDeviceToDevice, s_));
     return;
   }
-  gravity_compute();
+  gravity_pack();
+  hipStream_t gs = s_;
+  if (async) {
+    CS_HIP(hipEventRecord(ev_fork_, s_));
+    CS_HIP(hipStreamWaitEvent(gs_, ev_fork_, 0));
+    gs = gs_;
+  }
+  gravity_kernels(gs);
   if (adj_mode_ == kAdjRecord)
     CS_HIP(hipMemcpyAsync(adj_grav_.p + (size_t)adj_f_ * nd_, grav_, (size_t)nd_ * sizeof(V4<T>),
-                          hipMemcpyDeviceToDevice, s_));
+                          hipMemcpyDeviceToDevice, gs));
+  if (async) {
+    CS_HIP(hipEventRecord(ev_join_, gs_));
+    grav_pending_ = true;
+  }
 }
 
 template <class T>
-void Engine<T>::gravity_compute() {
+void Engine<T>::gravity_join() {
+  if (!grav_pending_) return;
+  CS_HIP(hipStreamWaitEvent(s_, ev_join_, 0));
+  grav_pending_ = false;
+}
+
+template <class T>
+void Engine<T>::gravity_pack() {
   if constexpr (kF32) {
     if (mixed()) {
       launch_gravity_pack_mixed(pos64_, pos_, vel_, ghi_, glo_, gvel32_, n_, nd_, s_);
-      TimerScope ts(timers_, "gravity", s_);
-      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, nd_, nd_, p_.grav_mode, p_.grav_eps, grav_split_only_,
-                           s_);
       return;
     }
   }
   launch_gravity_pack<T>(pos_, vel_, gpos_, gvel_, n_, nd_, s_);
-  TimerScope ts(timers_, "gravity", s_);
-  launch_gravity<T>(gpos_, gvel_, grav_, nd_, nd_, p_.grav_mode, T(p_.grav_eps), exact_rsqrt_, s_);
+}
+
+template <class T>
+void Engine<T>::gravity_kernels(hipStream_t gs) {
+  TimerScope ts(timers_, "gravity", gs);
+  if constexpr (kF32) {
+    if (mixed()) {
+      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, nd_, nd_, p_.grav_mode, p_.grav_eps,
+                           grav_split_only_, gs);
+      return;
+    }
+  }
+  launch_gravity<T>(gpos_, gvel_, grav_, nd_, nd_, p_.grav_mode, T(p_.grav_eps), exact_rsqrt_, gs);
 }
 
 template <class T>
@@ -631,10 +667,12 @@ void Engine<T>::step() {
   // softening needs the new ranges and keeps that order.
   const bool grav_first = p_.grav_mode == 1;
   tree_fetch();            // also reads back dt_
-  if (grav_first) gravity();
+  if (grav_first) gravity(overlap_ != 0);
   tree_build();
   sph_after_tree();
-  if (!grav_first) gravity();
+  if (adj_capture_ >= 0) adj_capture(*adj_eval_[adj_capture_]);
+  if (!grav_first) gravity(false);
+  gravity_join();
   add_gravity();
   global_time_ += dt_;
   stats_pass();

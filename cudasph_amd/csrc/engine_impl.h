@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "engine.h"
@@ -34,7 +35,12 @@ class Engine final : public EngineBase {
  public:
   Engine(int form, const sph_params& p, int device);
   ~Engine() override {
+    if (s_) (void)hipStreamSynchronize(s_);
+    if (gs_) (void)hipStreamSynchronize(gs_);
     if (fetched_) (void)hipEventDestroy(fetched_);
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (gs_) (void)hipStreamDestroy(gs_);
     if (s_) (void)hipStreamDestroy(s_);
   }
   long set(const std::string& name, const void* host, long n_bytes) override;
@@ -96,8 +102,10 @@ class Engine final : public EngineBase {
   void hydro();
   void reaction();
   void sph_accelerations();
-  void gravity();
-  void gravity_compute();
+  void gravity(bool async = false);
+  void gravity_join();
+  void gravity_pack();
+  void gravity_kernels(hipStream_t gs);
   void add_gravity();
   void timestep(double* out);
   void timestep_compute(double* out);
@@ -118,7 +126,7 @@ class Engine final : public EngineBase {
  private:
   enum { kAdjOff = 0, kAdjRecord = 1, kAdjReplay = 2, kAdjReplayStoredGravity = 3 };
   void adj_forces_backward(bool seed_rho);
-  void adj_save(DBuf<V4<T>>* dst[4], long slot);
+  void adj_save(DBuf<V4<T>>* dst[4], long slot);   // pos, vel, hydro, bdots
   void adj_load(DBuf<V4<T>>* src[4], long slot);
   double adj_loss();
   int adj_mode_ = kAdjOff, adj_K_ = 0, adj_k_ = 1, adj_f_ = 0, adj_iter0_ = 0;
@@ -128,7 +136,18 @@ class Engine final : public EngineBase {
   DBuf<int> adj_nngb_;                               // h-solve counts of evaluation f
   DBuf<V4<T>> adj_grav_;                             // gravity of evaluation f
   DBuf<V4<T>> adj_ck_pos_, adj_ck_vel_, adj_ck_hyd_, adj_ck_bd_;     // checkpoints, every k steps
-  DBuf<V4<T>> adj_seg_pos_, adj_seg_vel_, adj_seg_hyd_, adj_seg_bd_; // pre-step states of one segment
+  // force-evaluation data of each step of one segment, captured during its
+  // recomputation: state at the evaluation, lists and their transpose
+  struct EvalSlot {
+    DBuf<V4<T>> pos, vel, hyd, dots;
+    DBuf<int> off, cnt, list, tr_start, tr_end;
+    DBuf<KV> pairs;
+    long tot = 0;
+  };
+  std::vector<std::unique_ptr<EvalSlot>> adj_eval_;
+  int adj_capture_ = -1;           // slot to capture into during step(), or -1
+  void adj_capture(EvalSlot& e);
+  void adj_restore(const EvalSlot& e);
   DBuf<V4<T>> adj_init_pos_, adj_init_vel_, adj_init_hyd_, adj_final_vel_;
   DBuf<V4<T>> ax_, av_, aa_;       // carried adjoints of the state: x, v, (a, udot)
   DBuf<double> au_;                // carried adjoint of u
@@ -141,7 +160,9 @@ class Engine final : public EngineBase {
   int n_, nd_;
   long cap_;
   hipStream_t s_ = nullptr;
-  hipEvent_t fetched_ = nullptr;
+  hipEvent_t fetched_ = nullptr, ev_fork_ = nullptr, ev_join_ = nullptr;
+  hipStream_t gs_ = nullptr;       // gravity stream (overlap of gravity with the SPH passes)
+  bool grav_pending_ = false;
   Timers timers_;
 
   DBuf<V4<T>> pos_, vel_, hydro_, dots_, bdots_, hydro2_, grav_, tree_, gpos_, gvel_, old_vu_, tmp4_;
@@ -162,6 +183,7 @@ class Engine final : public EngineBase {
 
   double dt_ = 0, global_time_ = 0;
   int iteration_ = 0, exact_rsqrt_ = 0;
+  int overlap_ = 0;           // 1: gravity on gs_ beside the SPH passes (Plummer softening)
   int grav_split_only_ = 0;   // mixed-form gravity: 1 = split (hi/lo) differences for every pair
   int i_order_ = 1;   // 1: lanes take particles in kd-tree heap order, 0: in memory order
   long ngb_tot_ = 0;
