@@ -183,7 +183,10 @@ class Engine final : public EngineBase {
 
   double dt_ = 0, global_time_ = 0;
   int iteration_ = 0, exact_rsqrt_ = 0;
-  int overlap_ = 0;           // 1: gravity on gs_ beside the SPH passes (Plummer softening)
+  // 1: gravity on gs_ beside the SPH passes (Plummer softening).  Off by
+  // default: measured at N = 2^20 the co-running SPH kernels slowed gravity
+  // by more than the SPH time they hid
+  int overlap_ = 0;
   int grav_split_only_ = 0;   // mixed-form gravity: 1 = split (hi/lo) differences for every pair
   int i_order_ = 1;   // 1: lanes take particles in kd-tree heap order, 0: in memory order
   long ngb_tot_ = 0;

@@ -192,10 +192,10 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
       if constexpr (SOFT == 0) pair_spline<T, EXACT>(a, rx, ry, rz, ri2, pj.w, s_r2[j]);
       else pair_plummer<T, EXACT>(a, rx, ry, rz, pj.w, eps2);
     };
-    // j stays in ascending order (the reference's summation order); full
-    // tiles have a fixed trip count so the loop unrolls and LDS reads are
-    // issued ahead of their use
-    if (cnt == kJTile) {
+    // j stays in ascending order (the reference's summation order).  The
+    // FP64 form unrolls full tiles; the FP32 reference-layout kernels keep the
+    // rolled loop, which measured faster for their longer (dreal) pair body
+    if (ACC == 2 && cnt == kJTile) {
 #pragma unroll 8
       for (int j = 0; j < kJTile; ++j) body(j);
     } else {
