@@ -208,22 +208,23 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
 // mixed form, Plummer: split positions, FMA pair arithmetic, blocked
 // compensated sums.  hi = (x,y,z,m) rounded, lo = (x-hi, y-hi, z-hi, 0).
 //
-// Two ways to form r = x_i - x_j, chosen per (wavefront, j-tile):
+// Two ways to form r = x_i - x_j, chosen per (wavefront, sub-tile of 64 j):
 //  split:  r = (hi_i - hi_j) + (lo_i - lo_j), 9 operations per pair, relative
 //          error ~2^-24 for every pair;
-//  tile:   the tile is expressed relative to its middle record c,
+//  tile:   the sub-tile is expressed relative to its middle record c,
 //          d_j = (hi_j - c) + lo_j (staged in LDS) and d_i = (hi_i - c) + lo_i
-//          (once per tile per lane), r = d_i - d_j: 3 operations per pair,
+//          (once per sub-tile per lane), r = d_i - d_j: 3 operations per pair,
 //          absolute error <= 2^-24 (|d_i| + |d_j|) + rounding of r.
 // The tile form is used only when every lane of the wavefront has
-// |d_i| >= 3 R_t, R_t = max_j |d_j|: then |r| >= |d_i| - R_t >= 2 R_t and
-// (|d_i| + |d_j|) / |r| <= (|d_i| + R_t) / (|d_i| - R_t) <= 2, so r keeps a
+// |d_i| >= 3 R, R = max_j |d_j| over the sub-tile: then |r| >= |d_i| - R >= 2 R
+// and (|d_i| + |d_j|) / |r| <= (|d_i| + R) / (|d_i| - R) <= 2, so r keeps a
 // relative error of a few 2^-24 -- the accuracy of the split form.
-// Wavefronts near a tile (the pairs whose separation the split coordinates
-// exist for) use the split form.  With particles in Peano-Hilbert order the
-// tiles are compact and most (wavefront, tile) combinations are far apart;
-// with an unordered input the tiles are as large as the system and every
-// wavefront uses the split form.  split_only = 1 forces the split form everywhere.
+// Wavefronts near a sub-tile (the pairs whose separation the split
+// coordinates exist for) use the split form.  With particles in
+// Peano-Hilbert order 64 consecutive records are compact and most
+// (wavefront, sub-tile) combinations are far apart; with an unordered input
+// every sub-tile is as large as the system and every wavefront uses the split
+// form.  split_only = 1 forces the split form everywhere.
 // After r: 3 FMAs for r^2 + eps^2, one rsq, 3 multiplies, 3 FMAs into the
 // partial acceleration, 1 subtraction for the potential.
 //
@@ -240,42 +241,44 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
                                                                        F4* __restrict__ part_lo, int n_i,
                                                                        int n_j, int j_chunk, float eps2,
                                                                        int split_only) {
+  constexpr int kSub = kJTile / kTile;   // sub-tiles of kTile records, each with its own origin
   __shared__ F4 s_hi[kJTile], s_lo[kJTile], s_rel[kJTile];
-  __shared__ float s_red[kGravBlock / 64];
+  __shared__ F4 s_c[kSub];
+  __shared__ float s_r2[kSub];
+  static_assert(kTile == 64 && kJTile == 2 * kGravBlock, "each wavefront stages two whole sub-tiles");
   const int part = blockIdx.x % kGravParts;
   const int i = (blockIdx.x / kGravParts) * kGravBlock + threadIdx.x;
   const int ii = i < n_i ? i : n_i - 1;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const F4 pi = hi[ii], li = lo[ii];
   GAcc<float, 1> a[4];
   const int j_begin = min(part * j_chunk, n_j), j_end = min(j_begin + j_chunk, n_j);
   for (int t0 = j_begin; t0 < j_end; t0 += kJTile) {
     const int cnt = min(kJTile, j_end - t0);
-    const F4 c = hi[t0 + cnt / 2];   // the middle record: about half the tile's extent from either end
     __syncthreads();
-    float r2max = 0.f;
-    for (int q = threadIdx.x; q < cnt; q += kGravBlock) {
-      const F4 h = hi[t0 + q], l = lo[t0 + q];
-      const F4 d{(h.x - c.x) + l.x, (h.y - c.y) + l.y, (h.z - c.z) + l.z, h.w};
-      s_hi[q] = h;
-      s_lo[q] = l;
-      s_rel[q] = d;
-      r2max = fmaxf(r2max, __builtin_fmaf(d.x, d.x, __builtin_fmaf(d.y, d.y, d.z * d.z)));
+    // stage: this wavefront's two sub-tiles, each relative to its middle
+    // record, and each sub-tile's squared radius about that record
+    for (int q = threadIdx.x; q < kJTile; q += kGravBlock) {
+      const int sub = q / kTile, sub0 = sub * kTile;
+      float r2 = 0.f;
+      if (sub0 < cnt) {
+        const F4 c = hi[t0 + sub0 + min(kTile, cnt - sub0) / 2];
+        if (q < cnt) {
+          const F4 h = hi[t0 + q], l = lo[t0 + q];
+          const F4 d{(h.x - c.x) + l.x, (h.y - c.y) + l.y, (h.z - c.z) + l.z, h.w};
+          s_hi[q] = h;
+          s_lo[q] = l;
+          s_rel[q] = d;
+          r2 = __builtin_fmaf(d.x, d.x, __builtin_fmaf(d.y, d.y, d.z * d.z));
+        }
+        for (int o = 32; o > 0; o >>= 1) r2 = fmaxf(r2, __shfl_xor(r2, o));
+        if (lane == 0) {
+          s_c[sub] = c;
+          s_r2[sub] = r2;
+        }
+      }
     }
-    const float dx = (pi.x - c.x) + li.x, dy = (pi.y - c.y) + li.y, dz = (pi.z - c.z) + li.z;
-    float d2min = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-    for (int o = 32; o > 0; o >>= 1) {
-      r2max = fmaxf(r2max, __shfl_xor(r2max, o));
-      d2min = fminf(d2min, __shfl_xor(d2min, o));
-    }
-    if (lane == 0) s_red[wave] = r2max;
     __syncthreads();
-    float R2 = s_red[0];
-    for (int w = 1; w < kGravBlock / 64; ++w) R2 = fmaxf(R2, s_red[w]);
-    // |d_i| >= 3 R_t for every lane of this wave (the choice is uniform per
-    // wave, so there is no divergence); the squared test has a margin of 1.1
-    // for the rounding of the squares
-    const bool tile_form = !split_only && d2min >= 9.9f * R2;
     auto pair = [&](float rx, float ry, float rz, float mj) {
       const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
       const float inv = __builtin_amdgcn_rsqf(d2);
@@ -286,18 +289,26 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
       a[2].fsub(rz, s3);
       a[3].sub(mi);
     };
-    auto pair_tile = [&](int j) {
-      const F4 pj = s_rel[j];
-      pair(dx - pj.x, dy - pj.y, dz - pj.z, pj.w);
-    };
-    auto pair_split = [&](int j) {
-      const F4 pj = s_hi[j], lj = s_lo[j];
-      pair((pi.x - pj.x) + (li.x - lj.x), (pi.y - pj.y) + (li.y - lj.y), (pi.z - pj.z) + (li.z - lj.z), pj.w);
-    };
-    for (int j0 = 0; j0 < cnt; j0 += kTile) {
-      // full blocks (every block when n_j is a multiple of kTile) have a
+    for (int sub = 0; sub * kTile < cnt; ++sub) {
+      const int j0 = sub * kTile, nj = min(kTile, cnt - j0);
+      const F4 c = s_c[sub];
+      const float dx = (pi.x - c.x) + li.x, dy = (pi.y - c.y) + li.y, dz = (pi.z - c.z) + li.z;
+      // tile form only if every lane of the wavefront has |d_i| >= 3 R_sub
+      // (uniform per wavefront: no divergence); the squared test carries a
+      // margin of 1.1 for the rounding of the squares
+      const bool far = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) >= 9.9f * s_r2[sub];
+      const bool tile_form = !split_only && __all(far);
+      auto pair_tile = [&](int j) {
+        const F4 pj = s_rel[j];
+        pair(dx - pj.x, dy - pj.y, dz - pj.z, pj.w);
+      };
+      auto pair_split = [&](int j) {
+        const F4 pj = s_hi[j], lj = s_lo[j];
+        pair((pi.x - pj.x) + (li.x - lj.x), (pi.y - pj.y) + (li.y - lj.y), (pi.z - pj.z) + (li.z - lj.z), pj.w);
+      };
+      // full sub-tiles (all of them when n_j is a multiple of kTile) have a
       // fixed trip count, unrolled so LDS reads are issued ahead of their use
-      if (j0 + kTile <= cnt) {
+      if (nj == kTile) {
         if (tile_form) {
 #pragma unroll 16
           for (int j = 0; j < kTile; ++j) pair_tile(j0 + j);
@@ -306,7 +317,7 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
           for (int j = 0; j < kTile; ++j) pair_split(j0 + j);
         }
       } else {
-        for (int j = j0; j < cnt; ++j) {
+        for (int j = j0; j < j0 + nj; ++j) {
           if (tile_form) pair_tile(j);
           else pair_split(j);
         }

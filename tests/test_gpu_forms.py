@@ -81,7 +81,7 @@ def test_step_100_triggers_the_resort(cs, orc):
         assert_bits(e.get(name)[:n], o.get(name)[:n], name)
 
 
-@pytest.mark.parametrize("cfg", [2, 3])
+@pytest.mark.parametrize("cfg", [1, 2, 3, 5])
 def test_mixed_form_trajectory_against_fp64(cs, orc, cfg):
     """Three steps of the mixed form stay on the FP64 oracle's trajectory:
     energy within 1e-6 per step, positions and velocities to FP32 level."""
