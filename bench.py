@@ -253,7 +253,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-gravity-sample", type=int, default=256)
+    ap.add_argument("--cpu-gravity-sample", type=int, default=4096)
     ap.add_argument("--i-order", type=int, default=0, help="0: memory order, 1: kd-tree heap order")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-n", type=int, default=65536)

@@ -258,11 +258,22 @@ __global__ __launch_bounds__(kBlock) void k_reaction(const V4<T>* __restrict__ p
   if (j >= n) return;
   const V4<T> pj = pos[j];
   V4<T> d = bdots[j];
-  for (int p = start[j]; p < end[j]; ++p) {
-    const int k = (int)pairs[p].value;
-    const int i = owner[k];
-    const T acc = ngb_acc[k];
-    const V4<T> pi = pos[i];
+  const int p0 = start[j], p1 = end[j];
+  // software pipeline: the next slot's owner, coefficient and position are
+  // requested before the current term is added (the sum keeps slot order)
+  int kn = p0 < p1 ? (int)pairs[p0].value : 0;
+  int in_ = p0 < p1 ? owner[kn] : j;
+  T an = p0 < p1 ? ngb_acc[kn] : T(0);
+  V4<T> qn = pos[in_];
+  for (int p = p0; p < p1; ++p) {
+    const T acc = an;
+    const V4<T> pi = qn;
+    if (p + 1 < p1) {
+      kn = (int)pairs[p + 1].value;
+      in_ = owner[kn];
+      an = ngb_acc[kn];
+      qn = pos[in_];
+    }
     d.x += acc * (pi.x - pj.x);
     d.y += acc * (pi.y - pj.y);
     d.z += acc * (pi.z - pj.z);
@@ -294,11 +305,22 @@ __global__ __launch_bounds__(kBlock) void k_reaction_mixed(const F4* __restrict_
   const F4 pj = pos[j], lj = lo[j];
   F4 d = bdots[j];
   float sx = d.x, sy = d.y, sz = d.z, cx = 0.f, cy = 0.f, cz = 0.f;
-  for (int p = start[j]; p < end[j]; ++p) {
-    const int k = (int)pairs[p].value;
-    const int i = owner[k];
-    const float acc = ngb_acc[k];
-    const F4 pi = pos[i], li = lo[i];
+  const int p0 = start[j], p1 = end[j];
+  // software pipeline, as in k_reaction
+  int kn = p0 < p1 ? (int)pairs[p0].value : 0;
+  int in_ = p0 < p1 ? owner[kn] : j;
+  float an = p0 < p1 ? ngb_acc[kn] : 0.f;
+  F4 qn = pos[in_], ln = lo[in_];
+  for (int p = p0; p < p1; ++p) {
+    const float acc = an;
+    const F4 pi = qn, li = ln;
+    if (p + 1 < p1) {
+      kn = (int)pairs[p + 1].value;
+      in_ = owner[kn];
+      an = ngb_acc[kn];
+      qn = pos[in_];
+      ln = lo[in_];
+    }
     const float dx = (pi.x - pj.x) + (li.x - lj.x);
     const float dy = (pi.y - pj.y) + (li.y - lj.y);
     const float dz = (pi.z - pj.z) + (li.z - lj.z);

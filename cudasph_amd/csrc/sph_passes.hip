@@ -217,13 +217,12 @@ struct Sum<float, true> {
 // position difference r_i - r_j; the mixed form adds the low words of the
 // split coordinates
 template <class T, bool MIXED>
-__device__ __forceinline__ void pos_diff(const V4<T>& pi, const V4<T>& pj, const F4* lo, int i, int j,
-                                         const F4& loi, T& dx, T& dy, T& dz) {
+__device__ __forceinline__ void pos_diff(const V4<T>& pi, const V4<T>& pj, const F4& loi, const F4& lj, T& dx,
+                                         T& dy, T& dz) {
   dx = pi.x - pj.x;
   dy = pi.y - pj.y;
   dz = pi.z - pj.z;
   if constexpr (MIXED) {
-    const F4 lj = lo[j];
     dx += loi.x - lj.x;
     dy += loi.y - lj.y;
     dz += loi.z - lj.z;
@@ -245,11 +244,23 @@ __global__ __launch_bounds__(kBlock) void k_density(const V4<T>* __restrict__ po
   if constexpr (MIXED) loi = pos_lo[i];
   Sum<T, MIXED> rho, om, psi, den, divv, cx, cy, cz;
   const int off = offset[i], n = count[i];
+  // software pipeline: the next neighbour's id and records are requested
+  // before the current pair is computed (the sums keep list order)
+  int jn = n > 0 ? list[off] : i;
+  V4<T> pn = pos[jn], vn = vel[jn];
+  F4 ln{};
+  if constexpr (MIXED) ln = pos_lo[jn];
   for (int k = off; k < off + n; ++k) {
-    const int j = list[k];
-    const V4<T> pj = pos[j], vj = vel[j];
+    const V4<T> pj = pn, vj = vn;
+    const F4 lj = ln;
+    if (k + 1 < off + n) {
+      jn = list[k + 1];
+      pn = pos[jn];
+      vn = vel[jn];
+      if constexpr (MIXED) ln = pos_lo[jn];
+    }
     T dx, dy, dz;
-    pos_diff<T, MIXED>(bp, pj, pos_lo, i, j, loi, dx, dy, dz);
+    pos_diff<T, MIXED>(bp, pj, loi, lj, dx, dy, dz);
     const T u = T(2) * dsqrt(dx * dx + dy * dy + dz * dz) / bp.w;
     rho.add(vj.w * kern_w<T>(u));
     om.add(vj.w * kern_dwdh<T>(u));
@@ -321,11 +332,24 @@ __global__ __launch_bounds__(kBlock) void k_hydro(const V4<T>* __restrict__ pos,
   Sum<T, MIXED> ax, ay, az, ud;
   T vsig = T(0);
   const int off = offset[i], n = count[i];
+  // software pipeline, as in k_density
+  int jn = n > 0 ? list[off] : i;
+  V4<T> pn = pos[jn], vn = vel[jn], hn = hydro[jn], dn = dots[jn];
+  F4 ln{};
+  if constexpr (MIXED) ln = pos_lo[jn];
   for (int k = off; k < off + n; ++k) {
-    const int j = list[k];
-    const V4<T> pj = pos[j], vj = vel[j], hj = hydro[j], dj = dots[j];
+    const V4<T> pj = pn, vj = vn, hj = hn, dj = dn;
+    const F4 lj = ln;
+    if (k + 1 < off + n) {
+      jn = list[k + 1];
+      pn = pos[jn];
+      vn = vel[jn];
+      hn = hydro[jn];
+      dn = dots[jn];
+      if constexpr (MIXED) ln = pos_lo[jn];
+    }
     T dx, dy, dz;
-    pos_diff<T, MIXED>(bp, pj, pos_lo, i, j, loi, dx, dy, dz);
+    pos_diff<T, MIXED>(bp, pj, loi, lj, dx, dy, dz);
     const T dvx = bv.x - vj.x, dvy = bv.y - vj.y, dvz = bv.z - vj.z;
     const T ds = dsqrt(dx * dx + dy * dy + dz * dz);
     const T u = T((double)ds / hw);
