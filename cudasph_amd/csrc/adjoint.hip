@@ -425,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_gravity(const D4v* __restrict__ 
     }
     __syncthreads();
     const int m = min(kAdjTile, n - j0);
-    for (int jj = 0; jj < m; ++jj) {
+    auto body = [&](int jj) {
       const double rx = pi.x - sx[jj], ry = pi.y - sy[jj], rz = pi.z - sz[jj];
       const double d2 = rx * rx + ry * ry + rz * rz + eps2;
       double inv = __builtin_amdgcn_rsq(d2);
@@ -438,7 +438,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_gravity(const D4v* __restrict__ 
       gx += bx * inv3 - t * rx;
       gy += by * inv3 - t * ry;
       gz += bz * inv3 - t * rz;
-    }
+    };
+    // (unrolling this loop measured slower: the FP64 body is long enough)
+    for (int jj = 0; jj < m; ++jj) body(jj);
   }
   if (i < n) {
     D4v x = xs[i];
