@@ -192,15 +192,10 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
       if constexpr (SOFT == 0) pair_spline<T, EXACT>(a, rx, ry, rz, ri2, pj.w, s_r2[j]);
       else pair_plummer<T, EXACT>(a, rx, ry, rz, pj.w, eps2);
     };
-    // j stays in ascending order (the reference's summation order).  The
-    // FP64 form unrolls full tiles; the FP32 reference-layout kernels keep the
-    // rolled loop, which measured faster for their longer (dreal) pair body
-    if (ACC == 2 && cnt == kJTile) {
-#pragma unroll 8
-      for (int j = 0; j < kJTile; ++j) body(j);
-    } else {
-      for (int j = 0; j < cnt; ++j) body(j);
-    }
+    // j in ascending order (the reference's summation order).  The loop is
+    // left rolled: unrolling it measured slower for these longer pair bodies
+    // (dreal accumulation, FP64)
+    for (int j = 0; j < cnt; ++j) body(j);
   }
   if (i < n_i) out[i] = {a[0].get(), a[1].get(), a[2].get(), a[3].get()};
 }
