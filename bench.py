@@ -259,7 +259,7 @@ def main():
     ap.add_argument("--adjoint-n", type=int, default=65536)
     ap.add_argument("--adjoint-steps", type=int, default=20)
     ap.add_argument("--adjoint-ckpt", type=int, default=5)
-    ap.add_argument("--fd-dirs", type=int, default=8,
+    ap.add_argument("--fd-dirs", type=int, default=64,
                     help="seeded unit directions for the finite-difference check of the gradient")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the reverse-mode leg")
     ap.add_argument("--scale", action="store_true",

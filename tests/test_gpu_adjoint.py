@@ -26,9 +26,10 @@ def flat(gx, gv, gu):
     return np.concatenate([gx.ravel(), gv.ravel(), gu.ravel()])
 
 
-@pytest.mark.parametrize("K,ck", [(4, 3), (0, 1)])
-def test_adjoint_matches_fp64_cpu_reference_gradient(cs, K, ck):
-    n = 1024
+@pytest.mark.parametrize("K,ck,n", [(4, 3, 1024), (0, 1, 1024), (3, 2, 1000)])
+def test_adjoint_matches_fp64_cpu_reference_gradient(cs, K, ck, n):
+    """n = 1000 is padded to 1024 records: the padding carries no mass and is
+    in no neighbour list."""
     print(f"adjoint vs torch reference n={n} K={K} ck={ck}", flush=True)
     e, p, (pos, vel, hyd) = fp64_engine(cs, n)
     loss = e.adjoint_forward(K, ck)

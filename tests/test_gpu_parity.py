@@ -102,6 +102,26 @@ def test_plain_steps_bit_exact(cs, orc, cfg, n):
         assert se[key] == pytest.approx(so[key], rel=1e-12, abs=1e-14), key
 
 
+def test_plummer_ten_steps_lists_bit_exact(cs, orc):
+    """The Plummer sphere over 10 steps (Courant 0.3), neighbour lists compared
+    slot by slot after every step, and the state after the last."""
+    n = 16384
+    print(f"plummer 10 steps n={n}", flush=True)
+    e, o, _ = make_pair(cs, orc, 2, n, seed=19)
+    e.set_scalar("exact_rsqrt", 1)
+    e.init()
+    o.init()
+    for k in range(10):
+        e.step()
+        o.step()
+        le, lo = lists_of(e, n), lists_of(o, n)
+        bad = [i for i in range(n) if not np.array_equal(le[i], lo[i])]
+        assert not bad, f"step {k}: {len(bad)} lists differ, first particle {bad[0]}"
+        assert e.scalar("dt") == o.scalar("dt"), f"step {k}: dt"
+    for name in ("pos", "vel", "hydro", "dots", "bdots", "hydro2", "grav"):
+        assert_bits(e.get(name)[:n], o.get(name)[:n], name)
+
+
 @pytest.mark.parametrize("cfg,n", [(2, 16384), (3, 16384)])
 def test_plain_gravity_hw_rsqrt(cs, orc, cfg, n):
     """The bench's gravity (hardware rsqrt, as the reference's rsqrtf) against
