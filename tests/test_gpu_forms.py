@@ -110,20 +110,24 @@ def test_mixed_form_trajectory_against_fp64(cs, orc, cfg):
     assert drift <= 1e-6, drift
 
 
-@pytest.mark.parametrize("cfg,ph", [(2, False), (3, False), (2, True), (3, True)])
-def test_mixed_form_passes_against_fp64_oracle(cs, orc, cfg, ph):
+@pytest.mark.parametrize("cfg,ph,grav_mode", [(2, False, 1), (3, False, 1), (2, True, 1), (3, True, 1),
+                                              (2, False, 0), (3, True, 0)])
+def test_mixed_form_passes_against_fp64_oracle(cs, orc, cfg, ph, grav_mode):
     """Density, EOS, hydro + reaction and gravity of the mixed form against the
     FP64 oracle run on the mixed form's own state (same ranges): rho and P to
     2e-6, accelerations to 1e-5 of max(|a|, 1e-3 rms).  In Peano-Hilbert order
-    the gravity kernel takes its tile-relative form for distant tiles."""
-    print(f"mixed passes cfg={cfg} ph_sort={ph}", flush=True)
+    the Plummer kernel takes its tile-relative form for distant sub-tiles;
+    grav_mode 0 is the reference's M4-spline softening at the SPH range."""
+    print(f"mixed passes cfg={cfg} ph_sort={ph} grav_mode={grav_mode}", flush=True)
     n = 16384
-    e = cs.make_engine("mixed", cfg, n, 41, ph_sort=ph)
+    p = cs.preset_params(cfg, n)
+    p.grav_mode = grav_mode
+    e = cs.make_engine("mixed", cfg, n, 41, params=p, ph_sort=ph)
     e.init()
     e.step()
     for pid in (cs.SPH_ACC, cs.GRAVITY):
         e.run_pass(pid)
-    p = cs.preset_params(cfg, n)
+    p = e.params.copy()
     o = orc.Oracle(64, p)
     P = e.get("pos").astype(np.float64)
     P[:n, :3] = e.get("pos64")[:n, :3]
