@@ -53,8 +53,14 @@ struct GAcc<float, 1> {                      // blocked compensation
   float part = 0.f;
   __device__ __forceinline__ void sub(float y) { part -= y; }
   __device__ __forceinline__ void fsub(float a, float b) { part = __builtin_fmaf(-a, b, part); }
+  // fold the block's partial into (hi, lo): Knuth's two-sum of hi and the
+  // partial, its exact error added to lo (lo stays a small FP32 sum of the
+  // rounding errors; no renormalisation is needed until get())
   __device__ __forceinline__ void tile_end() {
-    dr_add(v, part);
+    const float s = v.hi + part;
+    const float bp = s - v.hi;
+    v.lo += (v.hi - (s - bp)) + (part - bp);
+    v.hi = s;
     part = 0.f;
   }
   __device__ __forceinline__ float get() const { return v.hi + v.lo; }
@@ -317,7 +323,9 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
           else pair_split(j);
         }
       }
-      for (int k = 0; k < 4; ++k) a[k].tile_end();
+      // fold the FP32 partials every two sub-tiles (128 pairs) and at the end
+      if ((sub & 1) || j0 + kTile >= cnt)
+        for (int k = 0; k < 4; ++k) a[k].tile_end();
     }
   }
   if (i < n_i) {
