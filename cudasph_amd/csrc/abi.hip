@@ -196,10 +196,10 @@ float host_sph_accelerations(int n_bodies, int /*ngb_tot*/, float av_alpha, floa
     c.gamma = a.p.eos_gamma;
     c.temp_base = a.p.eos_temp_base;
     c.newton = a.p.eos_newton;
-    launch_density<float, false>(a.pos, a.vel, nullptr, (F4*)hydro_data, (F4*)dots_data, (const KV*)bodies_hash,
+    launch_density<float>(a.pos, a.vel, (F4*)hydro_data, (F4*)dots_data, (const KV*)bodies_hash,
                                  ngb_offset, a.n_ngb, ngb_list, n_bodies, nullptr);
     launch_eos<float>((F4*)hydro_data, n_bodies, c, nullptr);
-    launch_hydro<float, false>(a.pos, a.vel, nullptr, (const F4*)hydro_data, (const F4*)dots_data,
+    launch_hydro<float>(a.pos, a.vel, (const F4*)hydro_data, (const F4*)dots_data,
                                (F4*)bodies_dots, (F4*)hydro_data2, ngb_acc, (const KV*)bodies_hash, ngb_offset,
                                a.n_ngb, ngb_list, n_bodies, av_alpha, av_beta, nullptr);
     CS_HIP(hipDeviceSynchronize());

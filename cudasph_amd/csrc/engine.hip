@@ -389,11 +389,11 @@ void Engine<T>::density() {
   TimerScope ts(timers_, "density", s_);
   if constexpr (kF32) {
     if (mixed())
-      launch_density<float, true>(pos_, vel_, pos_lo_, hydro_, dots_, hash_, ngb_offset_, ngb_count_, list_, nd_, s_);
+      launch_density_mixed(pos_, vel_, pos_lo_, hydro_, dots_, hash_, ngb_offset_, ngb_count_, list_, nd_, s_);
     else
-      launch_density<float, false>(pos_, vel_, nullptr, hydro_, dots_, hash_, ngb_offset_, ngb_count_, list_, nd_, s_);
+      launch_density<float>(pos_, vel_, hydro_, dots_, hash_, ngb_offset_, ngb_count_, list_, nd_, s_);
   } else {
-    launch_density<double, false>(pos_, vel_, nullptr, hydro_, dots_, hash_, ngb_offset_, ngb_count_, list_, nd_, s_);
+    launch_density<double>(pos_, vel_, hydro_, dots_, hash_, ngb_offset_, ngb_count_, list_, nd_, s_);
   }
 }
 
@@ -403,7 +403,7 @@ void Engine<T>::eos() {
   const EosConst c = eos_const();
   if constexpr (kF32) {
     if (mixed()) {
-      launch_eos_mixed((F4*)hydro_.p, u64_, n_, c, s_);
+      launch_eos_mixed((F4*)hydro_.p, u64_, (const F4*)pos_.p, (const F4*)dots_.p, (F4*)hydro2_.p, n_, c, s_);
       return;
     }
   }
@@ -416,13 +416,13 @@ void Engine<T>::hydro() {
   const T a = T(p_.av_alpha), b = T(p_.av_beta);
   if constexpr (kF32) {
     if (mixed())
-      launch_hydro<float, true>(pos_, vel_, pos_lo_, hydro_, dots_, bdots_, hydro2_, acc_, hash_, ngb_offset_,
-                                ngb_count_, list_, nd_, a, b, s_);
+      launch_hydro_mixed(pos_, vel_, pos_lo_, hydro_, dots_, bdots_, hydro2_, acc_, hash_, ngb_offset_, ngb_count_,
+                         list_, nd_, a, b, s_);
     else
-      launch_hydro<float, false>(pos_, vel_, nullptr, hydro_, dots_, bdots_, hydro2_, acc_, hash_, ngb_offset_,
+      launch_hydro<float>(pos_, vel_, hydro_, dots_, bdots_, hydro2_, acc_, hash_, ngb_offset_,
                                  ngb_count_, list_, nd_, a, b, s_);
   } else {
-    launch_hydro<double, false>(pos_, vel_, nullptr, hydro_, dots_, bdots_, hydro2_, acc_, hash_, ngb_offset_,
+    launch_hydro<double>(pos_, vel_, hydro_, dots_, bdots_, hydro2_, acc_, hash_, ngb_offset_,
                                 ngb_count_, list_, nd_, a, b, s_);
   }
 }

@@ -24,17 +24,25 @@ template <class T>
 void launch_find_ngb(const V4<T>* pos, const V4<T>* tree, KV* hash, const int* offset, const int* cap_count,
                      int* count_out, int* list, int* flags, int n_dev, const WalkParams& wp, int clamp,
                      hipStream_t s);
-template <class T, bool MIXED>
-void launch_density(const V4<T>* pos, const V4<T>* vel, const F4* pos_lo, V4<T>* hydro, V4<T>* dots,
-                    const KV* hash, const int* offset, const int* count, const int* list, int n_dev,
-                    hipStream_t s);
+// plain and FP64 forms
+template <class T>
+void launch_density(const V4<T>* pos, const V4<T>* vel, V4<T>* hydro, V4<T>* dots, const KV* hash,
+                    const int* offset, const int* count, const int* list, int n_dev, hipStream_t s);
 template <class T>
 void launch_eos(V4<T>* hydro, int n_dev, const EosConst& c, hipStream_t s);
-void launch_eos_mixed(F4* hydro, const double* u64, int n, const EosConst& c, hipStream_t s);
-template <class T, bool MIXED>
-void launch_hydro(const V4<T>* pos, const V4<T>* vel, const F4* pos_lo, const V4<T>* hydro, const V4<T>* dots,
-                  V4<T>* bdots, V4<T>* hydro2, T* ngb_acc, const KV* hash, const int* offset, const int* count,
-                  const int* list, int n_dev, T alpha, T beta, hipStream_t s);
+// mixed form: FP64 pressure and the per-particle viscosity factors into hydro2.yzw
+void launch_eos_mixed(F4* hydro, const double* u64, const F4* pos, const F4* dots, F4* hydro2, int n,
+                      const EosConst& c, hipStream_t s);
+// mixed form: FP32 pair arithmetic (f32k), Kahan sums, split positions
+void launch_density_mixed(const F4* pos, const F4* vel, const F4* pos_lo, F4* hydro, F4* dots, const KV* hash,
+                          const int* offset, const int* count, const int* list, int n_dev, hipStream_t s);
+void launch_hydro_mixed(const F4* pos, const F4* vel, const F4* pos_lo, const F4* hydro, const F4* dots, F4* bdots,
+                        F4* hydro2, float* ngb_acc, const KV* hash, const int* offset, const int* count,
+                        const int* list, int n_dev, float alpha, float beta, hipStream_t s);
+template <class T>
+void launch_hydro(const V4<T>* pos, const V4<T>* vel, const V4<T>* hydro, const V4<T>* dots, V4<T>* bdots,
+                  V4<T>* hydro2, T* ngb_acc, const KV* hash, const int* offset, const int* count, const int* list,
+                  int n_dev, T alpha, T beta, hipStream_t s);
 
 // ---- gravity.hip ----
 template <class T>

@@ -123,6 +123,66 @@ __device__ __forceinline__ T kern_dphidh(T u) {     // softening d(phi)/dh, :92-
   return T(0);
 }
 
+// ---- the same family in FP32 with FMA polynomials (mixed form) -----------
+// u = r/h; the reference's 4 pi * (1/pi) products are folded to 4.  Used only
+// by the mixed form, whose pair arithmetic is FP32 by definition; the plain
+// form keeps the reference's double-promoted evaluation above.
+namespace f32k {
+constexpr float kIPi = 0.318309886183791f;
+__device__ __forceinline__ float w(float u) {
+  if (u < 1.f) return kIPi * __builtin_fmaf(u * u, __builtin_fmaf(0.75f, u, -1.5f), 1.f);
+  if (u < 2.f) {
+    const float t = 2.f - u;
+    return (0.25f * kIPi) * t * t * t;
+  }
+  return 0.f;
+}
+// W'(u)/u; iu = 1/u (used on the outer branch only, where u >= 1)
+__device__ __forceinline__ float dw(float u, float iu) {
+  if (u < 1.f) return kIPi * __builtin_fmaf(2.25f, u, -3.f);
+  if (u < 2.f) {
+    const float t = 2.f - u;
+    return (-0.75f * kIPi) * t * t * iu;
+  }
+  return 0.f;
+}
+__device__ __forceinline__ float dwdh(float u) {
+  if (u < 1.f) return kIPi * __builtin_fmaf(u * u, __builtin_fmaf(-4.5f, u, 7.5f), -3.f);
+  if (u < 2.f) return kIPi * __builtin_fmaf(u, __builtin_fmaf(u, __builtin_fmaf(1.5f, u, -7.5f), 12.f), -6.f);
+  return 0.f;
+}
+__device__ __forceinline__ float dphidh(float u) {
+  const float u2 = u * u;
+  if (u < 1.f) return __builtin_fmaf(u2, __builtin_fmaf(u2, __builtin_fmaf(-0.6f, u, 1.5f), -2.f), 1.4f);
+  if (u < 2.f)
+    return __builtin_fmaf(u2, __builtin_fmaf(u, __builtin_fmaf(u, __builtin_fmaf(0.2f, u, -1.5f), 4.f), -4.f), 1.6f);
+  return 0.f;
+}
+// 4 pi y^2 W(y)
+__device__ __forceinline__ float diw(float y) {
+  const float y2 = y * y;
+  if (y < 1.f) return 4.f * y2 * __builtin_fmaf(y2, __builtin_fmaf(0.75f, y, -1.5f), 1.f);
+  if (y < 2.f) {
+    const float t = 2.f - y;
+    return y2 * t * t * t;
+  }
+  return 0.f;
+}
+__device__ __forceinline__ float digdh(float u) {
+  if (u <= 0.f) return 0.f;
+  if (u < 1.f) return 4.f * u * diw(4.f * u);
+  if (u < 2.f) return -4.f * u * diw(4.f * (2.f - u));
+  return 0.f;
+}
+// digdh(u) / u^2; iu = 1/u
+__device__ __forceinline__ float dig(float u, float iu) {
+  if (u <= 0.f) return 0.f;
+  if (u < 1.f) return 4.f * iu * diw(4.f * u);
+  if (u < 2.f) return -4.f * iu * diw(4.f * (2.f - u));
+  return 0.f;
+}
+}  // namespace f32k
+
 // ---- gas + radiation EOS (dev_compute_pressure.cu:3-111) -----------------
 
 struct EosConst {

@@ -197,42 +197,20 @@ struct KSum {
   }
   __device__ __forceinline__ float get() const { return s; }
 };
-template <class T, bool MIXED>
-struct Sum;
+// plain running sum in list order (the reference's accumulation)
 template <class T>
-struct Sum<T, false> {
+struct Sum {
   T s = T(0);
   __device__ __forceinline__ void add(T x) { s += x; }
   __device__ __forceinline__ void sub(T x) { s -= x; }
   __device__ __forceinline__ T get() const { return s; }
 };
-template <>
-struct Sum<float, true> {
-  KSum k;
-  __device__ __forceinline__ void add(float x) { k.add(x); }
-  __device__ __forceinline__ void sub(float x) { k.add(-x); }
-  __device__ __forceinline__ float get() const { return k.get(); }
-};
 
-// position difference r_i - r_j; the mixed form adds the low words of the
-// split coordinates
-template <class T, bool MIXED>
-__device__ __forceinline__ void pos_diff(const V4<T>& pi, const V4<T>& pj, const F4& loi, const F4& lj, T& dx,
-                                         T& dy, T& dz) {
-  dx = pi.x - pj.x;
-  dy = pi.y - pj.y;
-  dz = pi.z - pj.z;
-  if constexpr (MIXED) {
-    dx += loi.x - lj.x;
-    dy += loi.y - lj.y;
-    dz += loi.z - lj.z;
-  }
-}
-
-// density, grad-h terms, div v and |curl v| (dev_density_loop.cu:29-111)
-template <class T, bool MIXED>
+// density, grad-h terms, div v and |curl v| (dev_density_loop.cu:29-111);
+// plain and FP64 forms
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_density(const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel,
-                                                     const F4* __restrict__ pos_lo, V4<T>* __restrict__ hydro,
+                                                     V4<T>* __restrict__ hydro,
                                                      V4<T>* __restrict__ dots, const KV* __restrict__ hash,
                                                      const int* __restrict__ offset, const int* __restrict__ count,
                                                      const int* __restrict__ list, int n_dev) {
@@ -240,27 +218,20 @@ __global__ __launch_bounds__(kBlock) void k_density(const V4<T>* __restrict__ po
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
   const V4<T> bp = pos[i], bv = vel[i];
-  F4 loi{};
-  if constexpr (MIXED) loi = pos_lo[i];
-  Sum<T, MIXED> rho, om, psi, den, divv, cx, cy, cz;
+  Sum<T> rho, om, psi, den, divv, cx, cy, cz;
   const int off = offset[i], n = count[i];
   // software pipeline: the next neighbour's id and records are requested
   // before the current pair is computed (the sums keep list order)
   int jn = n > 0 ? list[off] : i;
   V4<T> pn = pos[jn], vn = vel[jn];
-  F4 ln{};
-  if constexpr (MIXED) ln = pos_lo[jn];
   for (int k = off; k < off + n; ++k) {
     const V4<T> pj = pn, vj = vn;
-    const F4 lj = ln;
     if (k + 1 < off + n) {
       jn = list[k + 1];
       pn = pos[jn];
       vn = vel[jn];
-      if constexpr (MIXED) ln = pos_lo[jn];
     }
-    T dx, dy, dz;
-    pos_diff<T, MIXED>(bp, pj, loi, lj, dx, dy, dz);
+    const T dx = bp.x - pj.x, dy = bp.y - pj.y, dz = bp.z - pj.z;
     const T u = T(2) * dsqrt(dx * dx + dy * dy + dz * dz) / bp.w;
     rho.add(vj.w * kern_w<T>(u));
     om.add(vj.w * kern_dwdh<T>(u));
@@ -299,20 +270,37 @@ __global__ __launch_bounds__(kBlock) void k_eos(V4<T>* __restrict__ hydro, int n
   hydro[i].y = eos_record<T>(hydro[i], c);
 }
 
-// mixed form: P from the FP64 internal energy with the FP64 Newton root
+// mixed form: P from the FP64 internal energy with the FP64 Newton root, and
+// the per-particle factors the viscosity needs (dev_art_visc.cu:12-26), once
+// per particle instead of once per pair: hydro2 = {v_sig (hydro pass),
+// sound speed c = sqrt(5/3 P/rho), P/rho^2, Balsara |div v| / (|div v| +
+// |curl v| + 1e-4 c/h)}
 __global__ __launch_bounds__(kBlock) void k_eos_mixed(F4* __restrict__ hydro, const double* __restrict__ u64,
-                                                       int n, EosConst c) {
+                                                       const F4* __restrict__ pos, const F4* __restrict__ dots,
+                                                       F4* __restrict__ hydro2, int n, EosConst c) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const F4 hd = hydro[i];
-  hydro[i].y = (float)eos_pressure_fp64((double)hd.x, u64[i], (double)hd.w, c);
+  const double P = eos_pressure_fp64((double)hd.x, u64[i], (double)hd.w, c);
+  hydro[i].y = (float)P;
+  const double rho = hd.x;
+  const double cs = sqrt(5.0 / 3.0 * P / rho);
+  const F4 dd = dots[i];
+  const double ad = fabs((double)dd.x);
+  const double bal = ad / ((ad + (double)dd.y) + 0.0001 * cs / (0.5 * (double)pos[i].w));
+  F4 h2 = hydro2[i];
+  h2.y = (float)cs;
+  h2.z = (float)(P / (rho * rho));
+  h2.w = (float)bal;
+  hydro2[i] = h2;
 }
 
 // hydro accelerations, du/dt and max signal speed (dev_hydro_dots.cu:25-113);
-// writes the per-pair reaction coefficient m_i*acc to ngb_acc[k]
-template <class T, bool MIXED>
+// writes the per-pair reaction coefficient m_i*acc to ngb_acc[k]; plain and
+// FP64 forms
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_hydro(const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel,
-                                                   const F4* __restrict__ pos_lo, const V4<T>* __restrict__ hydro,
+                                                   const V4<T>* __restrict__ hydro,
                                                    const V4<T>* __restrict__ dots, V4<T>* __restrict__ bdots,
                                                    V4<T>* __restrict__ hydro2, T* __restrict__ ngb_acc,
                                                    const KV* __restrict__ hash, const int* __restrict__ offset,
@@ -322,34 +310,27 @@ __global__ __launch_bounds__(kBlock) void k_hydro(const V4<T>* __restrict__ pos,
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
   const V4<T> bp = pos[i], bv = vel[i], hd = hydro[i], dd = dots[i];
-  F4 loi{};
-  if constexpr (MIXED) loi = pos_lo[i];
   const T prho2 = hd.y / (hd.x * hd.x);
   const double hw = 0.5 * (double)bp.w;
   // POW5(0.5*range) and POW2(0.5*range) expand left to right in the reference
   const double pow5 = hw * 0.5 * (double)bp.w * 0.5 * (double)bp.w * 0.5 * (double)bp.w * 0.5 * (double)bp.w;
   const double pow2 = hw * 0.5 * (double)bp.w;
-  Sum<T, MIXED> ax, ay, az, ud;
+  Sum<T> ax, ay, az, ud;
   T vsig = T(0);
   const int off = offset[i], n = count[i];
   // software pipeline, as in k_density
   int jn = n > 0 ? list[off] : i;
   V4<T> pn = pos[jn], vn = vel[jn], hn = hydro[jn], dn = dots[jn];
-  F4 ln{};
-  if constexpr (MIXED) ln = pos_lo[jn];
   for (int k = off; k < off + n; ++k) {
     const V4<T> pj = pn, vj = vn, hj = hn, dj = dn;
-    const F4 lj = ln;
     if (k + 1 < off + n) {
       jn = list[k + 1];
       pn = pos[jn];
       vn = vel[jn];
       hn = hydro[jn];
       dn = dots[jn];
-      if constexpr (MIXED) ln = pos_lo[jn];
     }
-    T dx, dy, dz;
-    pos_diff<T, MIXED>(bp, pj, loi, lj, dx, dy, dz);
+    const T dx = bp.x - pj.x, dy = bp.y - pj.y, dz = bp.z - pj.z;
     const T dvx = bv.x - vj.x, dvy = bv.y - vj.y, dvz = bv.z - vj.z;
     const T ds = dsqrt(dx * dx + dy * dy + dz * dz);
     const T u = T((double)ds / hw);
@@ -374,6 +355,139 @@ __global__ __launch_bounds__(kBlock) void k_hydro(const V4<T>* __restrict__ pos,
 }
 
 // ---------------------------------------------------------------------------
+// Mixed form: FP32 pair arithmetic with FMA polynomials (f32k), hardware
+// square root and reciprocals, split hi/lo position differences and Kahan
+// sums.  The per-particle quantities of the viscosity come precomputed from
+// k_eos_mixed.  Same sums as k_density / k_hydro, same list order.
+
+// density, grad-h terms, div v, |curl v| (dev_density_loop.cu:29-111)
+__global__ __launch_bounds__(kBlock) void k_density_mixed(const F4* __restrict__ pos, const F4* __restrict__ vel,
+                                                          const F4* __restrict__ pos_lo, F4* __restrict__ hydro,
+                                                          F4* __restrict__ dots, const KV* __restrict__ hash,
+                                                          const int* __restrict__ offset,
+                                                          const int* __restrict__ count,
+                                                          const int* __restrict__ list, int n_dev) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n_dev) return;
+  const int i = (int)hash[t].value;
+  const F4 bp = pos[i], bv = vel[i], loi = pos_lo[i];
+  const int off = offset[i], n = count[i];
+  if (n == 0) {
+    hydro[i].x = 0.f;
+    dots[i] = {0.f, 0.f, 0.f, 0.f};
+    return;
+  }
+  const float two_over_r = 2.f / bp.w;
+  KSum rho, om, psi, den, divv, cx, cy, cz;
+  int jn = list[off];
+  F4 pn = pos[jn], vn = vel[jn], ln = pos_lo[jn];
+  for (int k = off; k < off + n; ++k) {
+    const F4 pj = pn, vj = vn, lj = ln;
+    if (k + 1 < off + n) {
+      jn = list[k + 1];
+      pn = pos[jn];
+      vn = vel[jn];
+      ln = pos_lo[jn];
+    }
+    const float dx = (bp.x - pj.x) + (loi.x - lj.x);
+    const float dy = (bp.y - pj.y) + (loi.y - lj.y);
+    const float dz = (bp.z - pj.z) + (loi.z - lj.z);
+    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    const float u = __builtin_amdgcn_sqrtf(d2) * two_over_r;
+    const float iu = __builtin_amdgcn_rcpf(u);
+    rho.add(vj.w * f32k::w(u));
+    om.add(vj.w * f32k::dwdh(u));
+    psi.add(vj.w * f32k::dphidh(u));
+    den.add(-f32k::digdh(u));
+    const float dvx = bv.x - vj.x, dvy = bv.y - vj.y, dvz = bv.z - vj.z;
+    const float mdw = bv.w * f32k::dw(u, iu);
+    divv.add(mdw * __builtin_fmaf(dx, dvx, __builtin_fmaf(dy, dvy, dz * dvz)));
+    cx.add(mdw * __builtin_fmaf(dy, dvz, -dz * dvy));
+    cy.add(mdw * __builtin_fmaf(dz, dvx, -dx * dvz));
+    cz.add(mdw * __builtin_fmaf(dx, dvy, -dy * dvx));
+  }
+  const double h = 0.5 * (double)bp.w, h3 = h * h * h, h5 = h3 * h * h;
+  const double d = den.get();
+  double o = 0.0, ps = 0.0;
+  if (d != 0.0) {
+    o = om.get() / (h3 * d);
+    ps = psi.get() / (h * d);
+  }
+  const double r = rho.get() / h3;
+  const double x = cx.get(), y = cy.get(), z = cz.get();
+  hydro[i].x = (float)r;
+  dots[i] = {(float)(divv.get() / (r * h5)), (float)(sqrt(x * x + y * y + z * z) / (r * h5)), (float)o,
+             (float)ps};
+}
+
+// hydro accelerations, du/dt, v_sig and the reaction coefficients
+// (dev_hydro_dots.cu:25-113, dev_art_visc.cu:8-32)
+__global__ __launch_bounds__(kBlock) void k_hydro_mixed(const F4* __restrict__ pos, const F4* __restrict__ vel,
+                                                        const F4* __restrict__ pos_lo, const F4* __restrict__ hydro,
+                                                        const F4* __restrict__ dots, F4* __restrict__ bdots,
+                                                        F4* __restrict__ hydro2, float* __restrict__ ngb_acc,
+                                                        const KV* __restrict__ hash, const int* __restrict__ offset,
+                                                        const int* __restrict__ count, const int* __restrict__ list,
+                                                        int n_dev, float alpha, float beta) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n_dev) return;
+  const int i = (int)hash[t].value;
+  const int off = offset[i], n = count[i];
+  if (n == 0) {
+    bdots[i] = {0.f, 0.f, 0.f, 0.f};
+    hydro2[i].x = 0.f;
+    return;
+  }
+  const F4 bp = pos[i], bv = vel[i], loi = pos_lo[i], dd = dots[i], qi = hydro2[i];
+  const float ci = qi.y, prho2 = qi.z, bal_i = qi.w;
+  const float inv_h = 2.f / bp.w, inv_h2 = inv_h * inv_h, inv_h5 = inv_h2 * inv_h2 * inv_h;
+  const float k1 = (prho2 * dd.z + 0.5f * dd.w) * inv_h2;   // (P/rho^2 Omega + Psi/2) / h^2
+  const float k2 = prho2 * dd.z * inv_h2;                   // P/rho^2 Omega / h^2
+  KSum ax, ay, az, ud;
+  float vsig = 0.f;
+  int jn = list[off];
+  F4 pn = pos[jn], vn = vel[jn], ln = pos_lo[jn], qn = hydro2[jn];
+  for (int k = off; k < off + n; ++k) {
+    const F4 pj = pn, vj = vn, lj = ln, qj = qn;
+    if (k + 1 < off + n) {
+      jn = list[k + 1];
+      pn = pos[jn];
+      vn = vel[jn];
+      ln = pos_lo[jn];
+      qn = hydro2[jn];
+    }
+    const float dx = (bp.x - pj.x) + (loi.x - lj.x);
+    const float dy = (bp.y - pj.y) + (loi.y - lj.y);
+    const float dz = (bp.z - pj.z) + (loi.z - lj.z);
+    const float dvx = bv.x - vj.x, dvy = bv.y - vj.y, dvz = bv.z - vj.z;
+    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    const float ds = __builtin_amdgcn_sqrtf(d2);
+    const float u = ds * inv_h;
+    const float iu = __builtin_amdgcn_rcpf(u);
+    const float drdv = __builtin_fmaf(dx, dvx, __builtin_fmaf(dy, dvy, dz * dvz));
+    const float cs = ci + qj.y;
+    float pi_ = 0.f, vs = cs;
+    if (drdv < 0.f) {
+      const float r_ds = __builtin_amdgcn_rcpf(ds);
+      vs = __builtin_fmaf(-beta * drdv, r_ds, cs);
+      const float mu = drdv * r_ds * __builtin_amdgcn_rcpf(cs) * (bal_i + qj.w);
+      pi_ = (prho2 + qj.z) * mu * __builtin_fmaf(beta, mu, -alpha);
+    }
+    vsig = fmaxf(vsig, vs);
+    const float w1 = f32k::dw(u, iu), g = f32k::dig(u, iu);
+    const float t1 = __builtin_fmaf(0.5f, pi_, prho2) * w1 * inv_h5;
+    const float mjA = __builtin_fmaf(vj.w, t1, -k1 * g);      // m_j * acc
+    ax.add(-mjA * dx);
+    ay.add(-mjA * dy);
+    az.add(-mjA * dz);
+    ngb_acc[k] = bv.w * mjA * __builtin_amdgcn_rcpf(vj.w);  // m_i * acc
+    ud.add(__builtin_fmaf(vj.w, t1, -k2 * g) * drdv);
+  }
+  bdots[i] = {ax.get(), ay.get(), az.get(), ud.get()};
+  hydro2[i].x = vsig;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 template <class T>
 void launch_solve_range(V4<T>* pos, int* n_ngb, const KV* hash, const V4<T>* tree, int n_dev,
@@ -392,12 +506,11 @@ void launch_find_ngb(const V4<T>* pos, const V4<T>* tree, KV* hash, const int* o
   CS_CHECK_LAUNCH();
 }
 
-template <class T, bool MIXED>
-void launch_density(const V4<T>* pos, const V4<T>* vel, const F4* pos_lo, V4<T>* hydro, V4<T>* dots,
-                    const KV* hash, const int* offset, const int* count, const int* list, int n_dev,
-                    hipStream_t s) {
-  hipLaunchKernelGGL((k_density<T, MIXED>), dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, vel,
-                     pos_lo, hydro, dots, hash, offset, count, list, n_dev);
+template <class T>
+void launch_density(const V4<T>* pos, const V4<T>* vel, V4<T>* hydro, V4<T>* dots, const KV* hash,
+                    const int* offset, const int* count, const int* list, int n_dev, hipStream_t s) {
+  hipLaunchKernelGGL(k_density<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, vel, hydro, dots, hash,
+                     offset, count, list, n_dev);
   CS_CHECK_LAUNCH();
 }
 
@@ -407,17 +520,33 @@ void launch_eos(V4<T>* hydro, int n_dev, const EosConst& c, hipStream_t s) {
   CS_CHECK_LAUNCH();
 }
 
-void launch_eos_mixed(F4* hydro, const double* u64, int n, const EosConst& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_eos_mixed, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, hydro, u64, n, c);
+void launch_eos_mixed(F4* hydro, const double* u64, const F4* pos, const F4* dots, F4* hydro2, int n,
+                      const EosConst& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_eos_mixed, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, hydro, u64, pos, dots, hydro2, n, c);
   CS_CHECK_LAUNCH();
 }
 
-template <class T, bool MIXED>
-void launch_hydro(const V4<T>* pos, const V4<T>* vel, const F4* pos_lo, const V4<T>* hydro, const V4<T>* dots,
-                  V4<T>* bdots, V4<T>* hydro2, T* ngb_acc, const KV* hash, const int* offset, const int* count,
-                  const int* list, int n_dev, T alpha, T beta, hipStream_t s) {
-  hipLaunchKernelGGL((k_hydro<T, MIXED>), dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, vel, pos_lo,
-                     hydro, dots, bdots, hydro2, ngb_acc, hash, offset, count, list, n_dev, alpha, beta);
+void launch_density_mixed(const F4* pos, const F4* vel, const F4* pos_lo, F4* hydro, F4* dots, const KV* hash,
+                          const int* offset, const int* count, const int* list, int n_dev, hipStream_t s) {
+  hipLaunchKernelGGL(k_density_mixed, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, vel, pos_lo, hydro,
+                     dots, hash, offset, count, list, n_dev);
+  CS_CHECK_LAUNCH();
+}
+
+void launch_hydro_mixed(const F4* pos, const F4* vel, const F4* pos_lo, const F4* hydro, const F4* dots, F4* bdots,
+                        F4* hydro2, float* ngb_acc, const KV* hash, const int* offset, const int* count,
+                        const int* list, int n_dev, float alpha, float beta, hipStream_t s) {
+  hipLaunchKernelGGL(k_hydro_mixed, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, vel, pos_lo, hydro, dots,
+                     bdots, hydro2, ngb_acc, hash, offset, count, list, n_dev, alpha, beta);
+  CS_CHECK_LAUNCH();
+}
+
+template <class T>
+void launch_hydro(const V4<T>* pos, const V4<T>* vel, const V4<T>* hydro, const V4<T>* dots, V4<T>* bdots,
+                  V4<T>* hydro2, T* ngb_acc, const KV* hash, const int* offset, const int* count, const int* list,
+                  int n_dev, T alpha, T beta, hipStream_t s) {
+  hipLaunchKernelGGL(k_hydro<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, vel, hydro, dots, bdots,
+                     hydro2, ngb_acc, hash, offset, count, list, n_dev, alpha, beta);
   CS_CHECK_LAUNCH();
 }
 
@@ -430,20 +559,13 @@ void launch_hydro(const V4<T>* pos, const V4<T>* vel, const F4* pos_lo, const V4
 CS_INST(float)
 CS_INST(double)
 #undef CS_INST
-template void launch_density<float, false>(const F4*, const F4*, const F4*, F4*, F4*, const KV*, const int*,
-                                           const int*, const int*, int, hipStream_t);
-template void launch_density<float, true>(const F4*, const F4*, const F4*, F4*, F4*, const KV*, const int*,
-                                          const int*, const int*, int, hipStream_t);
-template void launch_density<double, false>(const D4*, const D4*, const F4*, D4*, D4*, const KV*, const int*,
-                                            const int*, const int*, int, hipStream_t);
-template void launch_hydro<float, false>(const F4*, const F4*, const F4*, const F4*, const F4*, F4*, F4*,
-                                         float*, const KV*, const int*, const int*, const int*, int, float,
-                                         float, hipStream_t);
-template void launch_hydro<float, true>(const F4*, const F4*, const F4*, const F4*, const F4*, F4*, F4*,
-                                        float*, const KV*, const int*, const int*, const int*, int, float,
-                                        float, hipStream_t);
-template void launch_hydro<double, false>(const D4*, const D4*, const F4*, const D4*, const D4*, D4*, D4*,
-                                          double*, const KV*, const int*, const int*, const int*, int, double,
-                                          double, hipStream_t);
+template void launch_density<float>(const F4*, const F4*, F4*, F4*, const KV*, const int*, const int*, const int*,
+                                    int, hipStream_t);
+template void launch_density<double>(const D4*, const D4*, D4*, D4*, const KV*, const int*, const int*, const int*,
+                                     int, hipStream_t);
+template void launch_hydro<float>(const F4*, const F4*, const F4*, const F4*, F4*, F4*, float*, const KV*, const int*,
+                                  const int*, const int*, int, float, float, hipStream_t);
+template void launch_hydro<double>(const D4*, const D4*, const D4*, const D4*, D4*, D4*, double*, const KV*,
+                                   const int*, const int*, const int*, int, double, double, hipStream_t);
 
 }  // namespace cudasph
