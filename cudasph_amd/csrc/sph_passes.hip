@@ -2,13 +2,15 @@
 // lists, density, EOS and hydro accelerations.
 //
 // One lane per particle.  Lanes take particles in the order of the hash
-// array (thread t -> bodies_hash[t].value, as in the reference); the engine
-// fills it with kd-tree heap order, so the 64 lanes of a wavefront hold
-// spatially adjacent particles and walk nearly the same nodes, and their
+// array (thread t -> bodies_hash[t].value, as in the reference).  The engine
+// fills it with kd-tree heap order, or with memory order when the particles
+// are already in Peano-Hilbert order; either way the 64 lanes of a wavefront
+// hold spatially adjacent particles, walk nearly the same nodes, and their
 // neighbour gathers hit the same cache lines.  The per-lane walk stack lives
 // in LDS, one column per lane (bank-conflict free).  Per-particle sums run in
-// list order, so each particle's result is the same bit pattern as the CPU
-// restatement's.
+// list order, so in the plain form each particle's result is the same bit
+// pattern as the CPU restatement's.  The mixed form has its own density and
+// hydro kernels (FP32 pair arithmetic, compensated sums).
 #include "kernels.h"
 #include "sph_math.h"
 
