@@ -121,6 +121,7 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
     glo_.alloc(nd_);
     gvel32_.alloc(nd_);
     gpart_.alloc(2 * 8 * (size_t)nd_);
+    gmass_.alloc(2 + 2 * (size_t)ceil_div(n_, kBlock));
     CS_HIP(hipMemsetAsync(pos_lo_, 0, pos_lo_.bytes(), s_));
     CS_HIP(hipMemsetAsync(pos64_, 0, pos64_.bytes(), s_));
     CS_HIP(hipMemsetAsync(vel64_, 0, vel64_.bytes(), s_));
@@ -526,8 +527,8 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
   TimerScope ts(timers_, "gravity", gs);
   if constexpr (kF32) {
     if (mixed()) {
-      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, nd_, nd_, p_.grav_mode, p_.grav_eps,
-                           grav_split_only_, gs);
+      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, gmass_, nd_, nd_, n_, p_.grav_mode,
+                           p_.grav_eps, grav_split_only_, gs);
       return;
     }
   }

@@ -175,6 +175,7 @@ class Engine final : public EngineBase {
   DBuf<D4> pos64_, vel64_, old64_;
   DBuf<double> u64_;
   DBuf<F4> pos_lo_, ghi_, glo_, gvel32_, gpart_;
+  DBuf<float> gmass_;   // {min, max} of the masses + reduction partials (mixed gravity)
   bool dirty64_ = false;
 
   HBuf<V4<T>> h_pos_, h_tree_;

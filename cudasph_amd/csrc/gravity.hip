@@ -241,7 +241,13 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
                                                                        F4* __restrict__ part_hi,
                                                                        F4* __restrict__ part_lo, int n_i,
                                                                        int n_j, int j_chunk, float eps2,
-                                                                       int split_only) {
+                                                                       int split_only,
+                                                                       const float* __restrict__ mass_range) {
+  // every mass equal (mass_range = {min, max} over the real particles): the
+  // pair body drops its mass factor and k_gravity_combine scales by m once;
+  // padding records sit at 1e30, where r^2 overflows and rsq returns 0, so
+  // they add exactly nothing without their zero mass
+  const bool uniform = mass_range[0] == mass_range[1];
   constexpr int kSub = kJTile / kTile;   // sub-tiles of kTile records, each with its own origin
   __shared__ F4 s_hi[kJTile], s_lo[kJTile], s_rel[kJTile];
   __shared__ F4 s_c[kSub];
@@ -290,6 +296,15 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
       a[2].fsub(rz, s3);
       a[3].sub(mi);
     };
+    auto pair_unit = [&](float rx, float ry, float rz) {   // unit mass: 12 VALU + rsq
+      const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+      const float inv = __builtin_amdgcn_rsqf(d2);
+      const float s3 = inv * (inv * inv);
+      a[0].fsub(rx, s3);
+      a[1].fsub(ry, s3);
+      a[2].fsub(rz, s3);
+      a[3].sub(inv);
+    };
     for (int sub = 0; sub * kTile < cnt; ++sub) {
       const int j0 = sub * kTile, nj = min(kTile, cnt - j0);
       const F4 c = s_c[sub];
@@ -307,10 +322,27 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
         const F4 pj = s_hi[j], lj = s_lo[j];
         pair((pi.x - pj.x) + (li.x - lj.x), (pi.y - pj.y) + (li.y - lj.y), (pi.z - pj.z) + (li.z - lj.z), pj.w);
       };
+      auto unit_tile = [&](int j) {
+        const F4 pj = s_rel[j];
+        pair_unit(dx - pj.x, dy - pj.y, dz - pj.z);
+      };
+      auto unit_split = [&](int j) {
+        const F4 pj = s_hi[j], lj = s_lo[j];
+        pair_unit((pi.x - pj.x) + (li.x - lj.x), (pi.y - pj.y) + (li.y - lj.y), (pi.z - pj.z) + (li.z - lj.z));
+      };
       // full sub-tiles (all of them when n_j is a multiple of kTile) have a
-      // fixed trip count, unrolled so LDS reads are issued ahead of their use
+      // fixed trip count, unrolled so LDS reads are issued ahead of their use;
+      // the branches are uniform per wavefront
       if (nj == kTile) {
-        if (tile_form) {
+        if (uniform) {
+          if (tile_form) {
+#pragma unroll 16
+            for (int j = 0; j < kTile; ++j) unit_tile(j0 + j);
+          } else {
+#pragma unroll 16
+            for (int j = 0; j < kTile; ++j) unit_split(j0 + j);
+          }
+        } else if (tile_form) {
 #pragma unroll 16
           for (int j = 0; j < kTile; ++j) pair_tile(j0 + j);
         } else {
@@ -319,8 +351,14 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
         }
       } else {
         for (int j = j0; j < j0 + nj; ++j) {
-          if (tile_form) pair_tile(j);
-          else pair_split(j);
+          if (uniform) {
+            if (tile_form) unit_tile(j);
+            else unit_split(j);
+          } else if (tile_form) {
+            pair_tile(j);
+          } else {
+            pair_split(j);
+          }
         }
       }
       // fold the FP32 partials every two sub-tiles (128 pairs) and at the end
@@ -335,12 +373,14 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
   }
 }
 
-// sum of the kGravParts compensated partials of each i, in part order
+// sum of the kGravParts compensated partials of each i, in part order,
+// times the common mass when every mass is equal
 __global__ __launch_bounds__(kBlock) void k_gravity_combine(const F4* __restrict__ part_hi,
                                                             const F4* __restrict__ part_lo, F4* __restrict__ out,
-                                                            int n_i) {
+                                                            int n_i, const float* __restrict__ mass_range) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n_i) return;
+  const float m = mass_range[0] == mass_range[1] ? mass_range[0] : 1.f;
   DR a[4] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
   for (int p = 0; p < kGravParts; ++p) {
     const F4 h = part_hi[(size_t)p * n_i + i], l = part_lo[(size_t)p * n_i + i];
@@ -353,7 +393,53 @@ __global__ __launch_bounds__(kBlock) void k_gravity_combine(const F4* __restrict
     dr_add(a[3], h.w);
     dr_add(a[3], l.w);
   }
-  out[i] = {a[0].hi + a[0].lo, a[1].hi + a[1].lo, a[2].hi + a[2].lo, a[3].hi + a[3].lo};
+  out[i] = {m * (a[0].hi + a[0].lo), m * (a[1].hi + a[1].lo), m * (a[2].hi + a[2].lo), m * (a[3].hi + a[3].lo)};
+}
+
+// {min, max} of the masses of the real particles: per-block partials, then
+// one block (fixed tree order, no atomics)
+__global__ __launch_bounds__(kBlock) void k_mass_range_partials(const F4* __restrict__ hi, int n,
+                                                                float* __restrict__ partial) {
+  __shared__ float smin[kBlock], smax[kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const float m = i < n ? hi[i].w : hi[0].w;
+  smin[threadIdx.x] = m;
+  smax[threadIdx.x] = m;
+  __syncthreads();
+  for (int k = kBlock / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + k]);
+      smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = smin[0];
+    partial[2 * blockIdx.x + 1] = smax[0];
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_mass_range_final(const float* __restrict__ partial, int nb,
+                                                             float* __restrict__ out) {
+  __shared__ float smin[kBlock], smax[kBlock];
+  float lo = partial[0], hi = partial[1];
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+    lo = fminf(lo, partial[2 * b]);
+    hi = fmaxf(hi, partial[2 * b + 1]);
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int k = kBlock / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + k]);
+      smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = smin[0];
+    out[1] = smax[0];
+  }
 }
 
 // mixed form, spline softening: same accumulation and split positions
@@ -410,20 +496,26 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
   CS_CHECK_LAUNCH();
 }
 
-void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, int n_i, int n_j,
-                          int soft, float eps, int split_only, hipStream_t s) {
+void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, float* mass_range,
+                          int n_i, int n_j, int n_real, int soft, float eps, int split_only, hipStream_t s) {
   const dim3 grid(ceil_div(n_i, kGravBlock)), block(kGravBlock);
   if (soft == 0) {
     hipLaunchKernelGGL(k_gravity_mixed_spline, grid, block, 0, s, hi, lo, gvel, out, n_i, n_j);
   } else {
+    // mass_range: 2 floats of result followed by 2 per block of n_real
+    const int nb = ceil_div(n_real, kBlock);
+    hipLaunchKernelGGL(k_mass_range_partials, dim3(nb), dim3(kBlock), 0, s, hi, n_real, mass_range + 2);
+    CS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_mass_range_final, dim3(1), dim3(kBlock), 0, s, mass_range + 2, nb, mass_range);
+    CS_CHECK_LAUNCH();
     const int chunk = ceil_div(ceil_div(n_j, kGravParts), kTile) * kTile;
     F4* part_hi = part;
     F4* part_lo = part + (size_t)kGravParts * n_i;
     hipLaunchKernelGGL(k_gravity_mixed_plummer, dim3(ceil_div(n_i, kGravBlock) * kGravParts), block, 0, s, hi, lo,
-                       part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only);
+                       part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range);
     CS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gravity_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part_hi, part_lo, out,
-                       n_i);
+                       n_i, mass_range);
   }
   CS_CHECK_LAUNCH();
 }
