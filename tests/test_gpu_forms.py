@@ -181,6 +181,40 @@ def test_mixed_gravity_forms_agree(cs):
     np.testing.assert_allclose(out[0][:, 3], out[1][:, 3], rtol=2e-6)
 
 
+@pytest.mark.parametrize("equal", [True, False])
+def test_mixed_gravity_against_fp64(cs, orc, equal):
+    """The mixed gravity pass against FP64 gravity on the same positions, with
+    equal masses (the unit-mass pair body) and with masses spread over a
+    factor of 3 (the general body); both Peano-Hilbert ordered, so both the
+    tile-relative and the split forms are taken."""
+    n = 16384
+    e = cs.make_engine("mixed", 2, n, 23, ph_sort=True)
+    v64 = e.get("vel64")
+    if not equal:
+        rng = np.random.default_rng(5)
+        v64[:n, 3] *= rng.uniform(0.5, 1.5, n)
+        e.set("vel64", v64)
+        vel = e.get("vel")
+        vel[:n, 3] = v64[:n, 3]
+        e.set("vel", vel)
+    e.run_pass(cs.GRAVITY)
+    g = e.get("grav")[:n].astype(np.float64)
+    p = e.params.copy()
+    o = orc.Oracle(64, p)
+    P = e.get("pos").astype(np.float64)
+    P[:n, :3] = e.get("pos64")[:n, :3]
+    V = e.get("vel").astype(np.float64)
+    V[:n] = v64[:n]
+    o.set("pos", P)
+    o.set("vel", V)
+    o.run_pass(orc.GRAVITY)
+    go = o.get("grav")[:n]
+    rms = np.sqrt((go[:, :3] ** 2).sum(1).mean())
+    err = np.linalg.norm(g[:, :3] - go[:, :3], axis=1) / np.maximum(np.linalg.norm(go[:, :3], axis=1), 1e-3 * rms)
+    assert err.max() < 1e-5, err.max()
+    np.testing.assert_allclose(g[:, 3], go[:, 3], rtol=2e-6)
+
+
 @pytest.mark.parametrize("form", ["plain", "mixed"])
 def test_gravity_stream_overlap_changes_nothing(cs, form):
     """Gravity on its own stream beside the SPH passes gives the same bits as
