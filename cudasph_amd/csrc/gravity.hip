@@ -225,7 +225,8 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
 // Peano-Hilbert order 64 consecutive records are compact and most
 // (wavefront, sub-tile) combinations are far apart; with an unordered input
 // every sub-tile is as large as the system and every wavefront uses the split
-// form.  split_only = 1 forces the split form everywhere.
+// form.  split_only bit 0 forces the split form everywhere; bit 1 disables
+// the unit-mass body (measurement switches).
 // After r: 3 FMAs for r^2 + eps^2, one rsq, 3 multiplies, 3 FMAs into the
 // partial acceleration, 1 subtraction for the potential.
 //
@@ -247,7 +248,8 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
   // pair body drops its mass factor and k_gravity_combine scales by m once;
   // padding records sit at 1e30, where r^2 overflows and rsq returns 0, so
   // they add exactly nothing without their zero mass
-  const bool uniform = mass_range[0] == mass_range[1];
+  const bool uniform = (split_only & 2) == 0 && mass_range[0] == mass_range[1];
+  split_only &= 1;
   constexpr int kSub = kJTile / kTile;   // sub-tiles of kTile records, each with its own origin
   __shared__ F4 s_hi[kJTile], s_lo[kJTile], s_rel[kJTile];
   __shared__ F4 s_c[kSub];
@@ -377,10 +379,11 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
 // times the common mass when every mass is equal
 __global__ __launch_bounds__(kBlock) void k_gravity_combine(const F4* __restrict__ part_hi,
                                                             const F4* __restrict__ part_lo, F4* __restrict__ out,
-                                                            int n_i, const float* __restrict__ mass_range) {
+                                                            int n_i, const float* __restrict__ mass_range,
+                                                            int flags) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n_i) return;
-  const float m = mass_range[0] == mass_range[1] ? mass_range[0] : 1.f;
+  const float m = (flags & 2) == 0 && mass_range[0] == mass_range[1] ? mass_range[0] : 1.f;
   DR a[4] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
   for (int p = 0; p < kGravParts; ++p) {
     const F4 h = part_hi[(size_t)p * n_i + i], l = part_lo[(size_t)p * n_i + i];
@@ -515,7 +518,7 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
                        part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range);
     CS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gravity_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part_hi, part_lo, out,
-                       n_i, mass_range);
+                       n_i, mass_range, split_only);
   }
   CS_CHECK_LAUNCH();
 }
