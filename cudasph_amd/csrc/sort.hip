@@ -293,56 +293,56 @@ template void launch_reaction<float>(const F4*, F4*, const float*, const KV*, co
 template void launch_reaction<double>(const D4*, D4*, const double*, const KV*, const int*, const int*,
                                       const int*, int, hipStream_t);
 
-// mixed form: split-coordinate differences, compensated sums
+// mixed form: the same gather with split positions and compensated sums; a
+// group of 8 lanes per j, each lane taking every 8th of j's slots, partial
+// sums combined with exact two-sums
 __global__ __launch_bounds__(kBlock) void k_reaction_mixed(const F4* __restrict__ pos, const F4* __restrict__ lo,
                                                             F4* __restrict__ bdots, const float* __restrict__ ngb_acc,
                                                             const KV* __restrict__ pairs,
                                                             const int* __restrict__ owner,
                                                             const int* __restrict__ start,
                                                             const int* __restrict__ end, int n) {
-  const int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= n) return;
+  constexpr int kG = 8;
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  const int j = g / kG, lane = g % kG;
+  if (j >= n) return;   // whole groups leave together
   const F4 pj = pos[j], lj = lo[j];
-  F4 d = bdots[j];
-  float sx = d.x, sy = d.y, sz = d.z, cx = 0.f, cy = 0.f, cz = 0.f;
-  const int p0 = start[j], p1 = end[j];
-  // software pipeline, as in k_reaction
-  int kn = p0 < p1 ? (int)pairs[p0].value : 0;
-  int in_ = p0 < p1 ? owner[kn] : j;
-  float an = p0 < p1 ? ngb_acc[kn] : 0.f;
-  F4 qn = pos[in_], ln = lo[in_];
-  for (int p = p0; p < p1; ++p) {
-    const float acc = an;
-    const F4 pi = qn, li = ln;
-    if (p + 1 < p1) {
-      kn = (int)pairs[p + 1].value;
-      in_ = owner[kn];
-      an = ngb_acc[kn];
-      qn = pos[in_];
-      ln = lo[in_];
-    }
-    const float dx = (pi.x - pj.x) + (li.x - lj.x);
-    const float dy = (pi.y - pj.y) + (li.y - lj.y);
-    const float dz = (pi.z - pj.z) + (li.z - lj.z);
-    float y = acc * dx - cx, t = sx + y;
-    cx = (t - sx) - y;
-    sx = t;
-    y = acc * dy - cy; t = sy + y;
-    cy = (t - sy) - y;
-    sy = t;
-    y = acc * dz - cz; t = sz + y;
-    cz = (t - sz) - y;
-    sz = t;
+  float s[3] = {0.f, 0.f, 0.f}, c[3] = {0.f, 0.f, 0.f};
+  auto kadd = [&](int q, float x) {
+    const float y = x - c[q];
+    const float t = s[q] + y;
+    c[q] = (t - s[q]) - y;
+    s[q] = t;
+  };
+  for (int p = start[j] + lane; p < end[j]; p += kG) {
+    const int k = (int)pairs[p].value;
+    const int i = owner[k];
+    const float acc = ngb_acc[k];
+    const F4 pi = pos[i], li = lo[i];
+    kadd(0, acc * ((pi.x - pj.x) + (li.x - lj.x)));
+    kadd(1, acc * ((pi.y - pj.y) + (li.y - lj.y)));
+    kadd(2, acc * ((pi.z - pj.z) + (li.z - lj.z)));
   }
-  d.x = sx;
-  d.y = sy;
-  d.z = sz;
+  for (int q = 0; q < 3; ++q)
+    for (int o = kG / 2; o > 0; o >>= 1) {
+      const float s2 = __shfl_xor(s[q], o, kG), c2 = __shfl_xor(c[q], o, kG);
+      const float t = s[q] + s2;
+      const float bp = t - s[q];
+      const float err = (s[q] - (t - bp)) + (s2 - bp);
+      c[q] = (c[q] + c2) - err;
+      s[q] = t;
+    }
+  if (lane != 0) return;
+  F4 d = bdots[j];
+  d.x += s[0] - c[0];
+  d.y += s[1] - c[1];
+  d.z += s[2] - c[2];
   bdots[j] = d;
 }
 
 void launch_reaction_mixed(const F4* pos, const F4* pos_lo, F4* bdots, const float* ngb_acc, const KV* pairs,
                            const int* owner, const int* start, const int* end, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_reaction_mixed, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, pos, pos_lo, bdots, ngb_acc,
+  hipLaunchKernelGGL(k_reaction_mixed, dim3(ceil_div(n * 8, kBlock)), dim3(kBlock), 0, s, pos, pos_lo, bdots, ngb_acc,
                      pairs, owner, start, end, n);
   CS_CHECK_LAUNCH();
 }

@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_find_ngb(const V4<T>* __restrict__ p
 // ---------------------------------------------------------------------------
 // compensated (two-sum) accumulator for the mixed form
 struct KSum {
-  float s = 0.f, c = 0.f;
+  float s = 0.f, c = 0.f;   // the compensated value is s - c
   __device__ __forceinline__ void add(float x) {
     const float y = x - c;
     const float t = s + y;
@@ -198,7 +198,29 @@ struct KSum {
     s = t;
   }
   __device__ __forceinline__ float get() const { return s; }
+  __device__ __forceinline__ float value() const { return s - c; }
 };
+
+// lanes per particle in the cooperative mixed-form kernels
+constexpr int kGroup = 8;
+
+// sum of the group's KSums into every lane of the group: a butterfly over
+// the kGroup lanes, each step an exact two-sum of the partial values with the
+// rounding error carried into the compensation
+__device__ __forceinline__ void group_sum(KSum& a) {
+  for (int o = kGroup / 2; o > 0; o >>= 1) {
+    const float s2 = __shfl_xor(a.s, o, kGroup), c2 = __shfl_xor(a.c, o, kGroup);
+    const float t = a.s + s2;
+    const float bp = t - a.s;
+    const float err = (a.s - (t - bp)) + (s2 - bp);   // a.s + s2 = t + err exactly
+    a.c = (a.c + c2) - err;
+    a.s = t;
+  }
+}
+__device__ __forceinline__ float group_max(float v) {
+  for (int o = kGroup / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kGroup));
+  return v;
+}
 // plain running sum in list order (the reference's accumulation)
 template <class T>
 struct Sum {
@@ -362,35 +384,26 @@ __global__ __launch_bounds__(kBlock) void k_hydro(const V4<T>* __restrict__ pos,
 // sums.  The per-particle quantities of the viscosity come precomputed from
 // k_eos_mixed.  Same sums as k_density / k_hydro, same list order.
 
-// density, grad-h terms, div v, |curl v| (dev_density_loop.cu:29-111)
+// density, grad-h terms, div v, |curl v| (dev_density_loop.cu:29-111); a
+// group of kGroup lanes per particle, each lane taking every kGroup-th
+// neighbour, partial sums combined by group_sum
 __global__ __launch_bounds__(kBlock) void k_density_mixed(const F4* __restrict__ pos, const F4* __restrict__ vel,
                                                           const F4* __restrict__ pos_lo, F4* __restrict__ hydro,
                                                           F4* __restrict__ dots, const KV* __restrict__ hash,
                                                           const int* __restrict__ offset,
                                                           const int* __restrict__ count,
                                                           const int* __restrict__ list, int n_dev) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= n_dev) return;
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  const int t = g / kGroup, lane = g % kGroup;
+  if (t >= n_dev) return;   // whole groups leave together
   const int i = (int)hash[t].value;
   const F4 bp = pos[i], bv = vel[i], loi = pos_lo[i];
   const int off = offset[i], n = count[i];
-  if (n == 0) {
-    hydro[i].x = 0.f;
-    dots[i] = {0.f, 0.f, 0.f, 0.f};
-    return;
-  }
-  const float two_over_r = 2.f / bp.w;
+  const float two_over_r = n > 0 ? 2.f / bp.w : 0.f;
   KSum rho, om, psi, den, divv, cx, cy, cz;
-  int jn = list[off];
-  F4 pn = pos[jn], vn = vel[jn], ln = pos_lo[jn];
-  for (int k = off; k < off + n; ++k) {
-    const F4 pj = pn, vj = vn, lj = ln;
-    if (k + 1 < off + n) {
-      jn = list[k + 1];
-      pn = pos[jn];
-      vn = vel[jn];
-      ln = pos_lo[jn];
-    }
+  for (int k = off + lane; k < off + n; k += kGroup) {
+    const int j = list[k];
+    const F4 pj = pos[j], vj = vel[j], lj = pos_lo[j];
     const float dx = (bp.x - pj.x) + (loi.x - lj.x);
     const float dy = (bp.y - pj.y) + (loi.y - lj.y);
     const float dz = (bp.z - pj.z) + (loi.z - lj.z);
@@ -408,22 +421,37 @@ __global__ __launch_bounds__(kBlock) void k_density_mixed(const F4* __restrict__
     cy.add(mdw * __builtin_fmaf(dz, dvx, -dx * dvz));
     cz.add(mdw * __builtin_fmaf(dx, dvy, -dy * dvx));
   }
+  group_sum(rho);
+  group_sum(om);
+  group_sum(psi);
+  group_sum(den);
+  group_sum(divv);
+  group_sum(cx);
+  group_sum(cy);
+  group_sum(cz);
+  if (lane != 0) return;
+  if (n == 0) {
+    hydro[i].x = 0.f;
+    dots[i] = {0.f, 0.f, 0.f, 0.f};
+    return;
+  }
   const double h = 0.5 * (double)bp.w, h3 = h * h * h, h5 = h3 * h * h;
-  const double d = den.get();
+  const double d = den.value();
   double o = 0.0, ps = 0.0;
   if (d != 0.0) {
-    o = om.get() / (h3 * d);
-    ps = psi.get() / (h * d);
+    o = om.value() / (h3 * d);
+    ps = psi.value() / (h * d);
   }
-  const double r = rho.get() / h3;
-  const double x = cx.get(), y = cy.get(), z = cz.get();
+  const double r = rho.value() / h3;
+  const double x = cx.value(), y = cy.value(), z = cz.value();
   hydro[i].x = (float)r;
-  dots[i] = {(float)(divv.get() / (r * h5)), (float)(sqrt(x * x + y * y + z * z) / (r * h5)), (float)o,
+  dots[i] = {(float)(divv.value() / (r * h5)), (float)(sqrt(x * x + y * y + z * z) / (r * h5)), (float)o,
              (float)ps};
 }
 
 // hydro accelerations, du/dt, v_sig and the reaction coefficients
-// (dev_hydro_dots.cu:25-113, dev_art_visc.cu:8-32)
+// (dev_hydro_dots.cu:25-113, dev_art_visc.cu:8-32); a group of kGroup lanes
+// per particle, as in k_density_mixed
 __global__ __launch_bounds__(kBlock) void k_hydro_mixed(const F4* __restrict__ pos, const F4* __restrict__ vel,
                                                         const F4* __restrict__ pos_lo, const F4* __restrict__ hydro,
                                                         const F4* __restrict__ dots, F4* __restrict__ bdots,
@@ -431,33 +459,21 @@ __global__ __launch_bounds__(kBlock) void k_hydro_mixed(const F4* __restrict__ p
                                                         const KV* __restrict__ hash, const int* __restrict__ offset,
                                                         const int* __restrict__ count, const int* __restrict__ list,
                                                         int n_dev, float alpha, float beta) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= n_dev) return;
+  const int gl = blockIdx.x * kBlock + threadIdx.x;
+  const int t = gl / kGroup, lane = gl % kGroup;
+  if (t >= n_dev) return;   // whole groups leave together
   const int i = (int)hash[t].value;
   const int off = offset[i], n = count[i];
-  if (n == 0) {
-    bdots[i] = {0.f, 0.f, 0.f, 0.f};
-    hydro2[i].x = 0.f;
-    return;
-  }
   const F4 bp = pos[i], bv = vel[i], loi = pos_lo[i], dd = dots[i], qi = hydro2[i];
   const float ci = qi.y, prho2 = qi.z, bal_i = qi.w;
-  const float inv_h = 2.f / bp.w, inv_h2 = inv_h * inv_h, inv_h5 = inv_h2 * inv_h2 * inv_h;
+  const float inv_h = n > 0 ? 2.f / bp.w : 0.f, inv_h2 = inv_h * inv_h, inv_h5 = inv_h2 * inv_h2 * inv_h;
   const float k1 = (prho2 * dd.z + 0.5f * dd.w) * inv_h2;   // (P/rho^2 Omega + Psi/2) / h^2
   const float k2 = prho2 * dd.z * inv_h2;                   // P/rho^2 Omega / h^2
   KSum ax, ay, az, ud;
   float vsig = 0.f;
-  int jn = list[off];
-  F4 pn = pos[jn], vn = vel[jn], ln = pos_lo[jn], qn = hydro2[jn];
-  for (int k = off; k < off + n; ++k) {
-    const F4 pj = pn, vj = vn, lj = ln, qj = qn;
-    if (k + 1 < off + n) {
-      jn = list[k + 1];
-      pn = pos[jn];
-      vn = vel[jn];
-      ln = pos_lo[jn];
-      qn = hydro2[jn];
-    }
+  for (int k = off + lane; k < off + n; k += kGroup) {
+    const int j = list[k];
+    const F4 pj = pos[j], vj = vel[j], lj = pos_lo[j], qj = hydro2[j];
     const float dx = (bp.x - pj.x) + (loi.x - lj.x);
     const float dy = (bp.y - pj.y) + (loi.y - lj.y);
     const float dz = (bp.z - pj.z) + (loi.z - lj.z);
@@ -476,16 +492,22 @@ __global__ __launch_bounds__(kBlock) void k_hydro_mixed(const F4* __restrict__ p
       pi_ = (prho2 + qj.z) * mu * __builtin_fmaf(beta, mu, -alpha);
     }
     vsig = fmaxf(vsig, vs);
-    const float w1 = f32k::dw(u, iu), g = f32k::dig(u, iu);
+    const float w1 = f32k::dw(u, iu), gk = f32k::dig(u, iu);
     const float t1 = __builtin_fmaf(0.5f, pi_, prho2) * w1 * inv_h5;
-    const float mjA = __builtin_fmaf(vj.w, t1, -k1 * g);      // m_j * acc
+    const float mjA = __builtin_fmaf(vj.w, t1, -k1 * gk);      // m_j * acc
     ax.add(-mjA * dx);
     ay.add(-mjA * dy);
     az.add(-mjA * dz);
-    ngb_acc[k] = bv.w * mjA * __builtin_amdgcn_rcpf(vj.w);  // m_i * acc
-    ud.add(__builtin_fmaf(vj.w, t1, -k2 * g) * drdv);
+    ngb_acc[k] = bv.w * mjA * __builtin_amdgcn_rcpf(vj.w);   // m_i * acc
+    ud.add(__builtin_fmaf(vj.w, t1, -k2 * gk) * drdv);
   }
-  bdots[i] = {ax.get(), ay.get(), az.get(), ud.get()};
+  group_sum(ax);
+  group_sum(ay);
+  group_sum(az);
+  group_sum(ud);
+  vsig = group_max(vsig);
+  if (lane != 0) return;
+  bdots[i] = {ax.value(), ay.value(), az.value(), ud.value()};
   hydro2[i].x = vsig;
 }
 
@@ -530,16 +552,16 @@ void launch_eos_mixed(F4* hydro, const double* u64, const F4* pos, const F4* dot
 
 void launch_density_mixed(const F4* pos, const F4* vel, const F4* pos_lo, F4* hydro, F4* dots, const KV* hash,
                           const int* offset, const int* count, const int* list, int n_dev, hipStream_t s) {
-  hipLaunchKernelGGL(k_density_mixed, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, vel, pos_lo, hydro,
-                     dots, hash, offset, count, list, n_dev);
+  hipLaunchKernelGGL(k_density_mixed, dim3(ceil_div(n_dev * kGroup, kBlock)), dim3(kBlock), 0, s, pos, vel, pos_lo,
+                     hydro, dots, hash, offset, count, list, n_dev);
   CS_CHECK_LAUNCH();
 }
 
 void launch_hydro_mixed(const F4* pos, const F4* vel, const F4* pos_lo, const F4* hydro, const F4* dots, F4* bdots,
                         F4* hydro2, float* ngb_acc, const KV* hash, const int* offset, const int* count,
                         const int* list, int n_dev, float alpha, float beta, hipStream_t s) {
-  hipLaunchKernelGGL(k_hydro_mixed, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, vel, pos_lo, hydro, dots,
-                     bdots, hydro2, ngb_acc, hash, offset, count, list, n_dev, alpha, beta);
+  hipLaunchKernelGGL(k_hydro_mixed, dim3(ceil_div(n_dev * kGroup, kBlock)), dim3(kBlock), 0, s, pos, vel, pos_lo,
+                     hydro, dots, bdots, hydro2, ngb_acc, hash, offset, count, list, n_dev, alpha, beta);
   CS_CHECK_LAUNCH();
 }
 
