@@ -181,6 +181,36 @@ def test_mixed_gravity_forms_agree(cs):
     np.testing.assert_allclose(out[0][:, 3], out[1][:, 3], rtol=2e-6)
 
 
+@pytest.mark.parametrize("n", [100, 300, 1000])
+def test_mixed_form_small_sizes(cs, orc, n):
+    """Small N: most of the 8 j-parts of the gravity grid are empty, sub-tiles
+    are partial and padding is a large share of the records."""
+    p = cs.preset_params(2, n)
+    pos, vel, hyd = cs.generate_ic(2, n, 3)
+    e = cs.make_engine("mixed", 2, n, 3, ph_sort=True)
+    o = orc.Oracle(64, e.params.copy())
+    P = e.get("pos").astype(np.float64)
+    P[:n, :3] = e.get("pos64")[:n, :3]
+    V = e.get("vel64")
+    H = e.get("hydro").astype(np.float64)
+    H[:n, 2] = e.get("u64")[:n]
+    o.set("pos", P)
+    o.set("vel", V)
+    o.set("hydro", H)
+    e.init()
+    o.init()
+    for _ in range(2):
+        e.step()
+        o.step()
+    e0 = abs(o.stats()["E"])
+    assert abs(e.stats()["E"] - o.stats()["E"]) <= 1e-6 * e0
+    g, go = e.get("grav")[:n, :3].astype(np.float64), o.get("grav")[:n, :3]
+    rms = np.sqrt((go ** 2).sum(1).mean())
+    err = np.linalg.norm(g - go, axis=1) / np.maximum(np.linalg.norm(go, axis=1), 1e-3 * rms)
+    assert err.max() < 1e-4, err.max()
+    assert np.isfinite(e.get("bdots")[:n]).all()
+
+
 @pytest.mark.parametrize("equal", [True, False])
 def test_mixed_gravity_against_fp64(cs, orc, equal):
     """The mixed gravity pass against FP64 gravity on the same positions, with
