@@ -236,6 +236,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "i_order") return i_order_;
   if (name == "grav_split_only") return grav_split_only_;
   if (name == "grav_unit_mass") return grav_unit_mass_;
+  if (name == "grav_expand") return grav_expand_;
   if (name == "overlap") return overlap_;
   if (name == "timing") return timers_.enabled;
   if (name == "cap") return (double)cap_;
@@ -262,6 +263,8 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     grav_split_only_ = (int)v;
   } else if (name == "grav_unit_mass") {
     grav_unit_mass_ = (int)v;
+  } else if (name == "grav_expand") {
+    grav_expand_ = (int)v;
   } else if (name == "overlap") {
     overlap_ = (int)v;
   } else if (name == "timing") {
@@ -531,7 +534,7 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
   if constexpr (kF32) {
     if (mixed()) {
       launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, gmass_, nd_, nd_, n_, p_.grav_mode,
-                           p_.grav_eps, grav_split_only_ | (grav_unit_mass_ ? 0 : 2), gs);
+                           p_.grav_eps, grav_split_only_ | (grav_unit_mass_ ? 0 : 2) | (grav_expand_ ? 0 : 4), gs);
       return;
     }
   }

@@ -189,7 +189,8 @@ class Engine final : public EngineBase {
   // by more than the SPH time they hid
   int overlap_ = 0;
   int grav_split_only_ = 0;
-  int grav_unit_mass_ = 1;    // mixed gravity: unit-mass pair body when every mass is equal   // mixed-form gravity: 1 = split (hi/lo) differences for every pair
+  int grav_unit_mass_ = 1;    // mixed gravity: unit-mass pair body when every mass is equal
+  int grav_expand_ = 1;       // mixed gravity: expanded r^2 in the equal-mass tile body
   int i_order_ = 1;   // 1: lanes take particles in kd-tree heap order, 0: in memory order
   long ngb_tot_ = 0;
   int overflow_reruns_ = 0;

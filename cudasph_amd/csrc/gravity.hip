@@ -226,9 +226,11 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
 // (wavefront, sub-tile) combinations are far apart; with an unordered input
 // every sub-tile is as large as the system and every wavefront uses the split
 // form.  split_only bit 0 forces the split form everywhere; bit 1 disables
-// the unit-mass body (measurement switches).
+// the unit-mass body; bit 2 the expanded unit-mass tile body (measurement
+// switches).
 // After r: 3 FMAs for r^2 + eps^2, one rsq, 3 multiplies, 3 FMAs into the
-// partial acceleration, 1 subtraction for the potential.
+// partial acceleration, 1 subtraction for the potential.  With every mass
+// equal, tile-form sub-tiles skip r altogether (the expanded body below).
 //
 // Grid: kGravParts j-parts per i-block, block b = i_block * kGravParts + part.
 // Workgroups are dispatched round-robin over the 8 XCDs, so part p runs on
@@ -249,6 +251,7 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
   // padding records sit at 1e30, where r^2 overflows and rsq returns 0, so
   // they add exactly nothing without their zero mass
   const bool uniform = (split_only & 2) == 0 && mass_range[0] == mass_range[1];
+  const bool expand = uniform && (split_only & 4) == 0;
   split_only &= 1;
   constexpr int kSub = kJTile / kTile;   // sub-tiles of kTile records, each with its own origin
   __shared__ F4 s_hi[kJTile], s_lo[kJTile], s_rel[kJTile];
@@ -274,11 +277,12 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
         const F4 c = hi[t0 + sub0 + min(kTile, cnt - sub0) / 2];
         if (q < cnt) {
           const F4 h = hi[t0 + q], l = lo[t0 + q];
-          const F4 d{(h.x - c.x) + l.x, (h.y - c.y) + l.y, (h.z - c.z) + l.z, h.w};
+          F4 d{(h.x - c.x) + l.x, (h.y - c.y) + l.y, (h.z - c.z) + l.z, h.w};
+          r2 = __builtin_fmaf(d.x, d.x, __builtin_fmaf(d.y, d.y, d.z * d.z));
+          if (expand) d.w = r2;   // the expanded body needs |d_j|^2, not m
           s_hi[q] = h;
           s_lo[q] = l;
           s_rel[q] = d;
-          r2 = __builtin_fmaf(d.x, d.x, __builtin_fmaf(d.y, d.y, d.z * d.z));
         }
         for (int o = 32; o > 0; o >>= 1) r2 = fmaxf(r2, __shfl_xor(r2, o));
         if (lane == 0) {
@@ -337,7 +341,32 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
       // the branches are uniform per wavefront
       if (nj == kTile) {
         if (uniform) {
-          if (tile_form) {
+          if (tile_form && expand) {
+            // expanded equal-mass body: r^2 + eps^2 = (|d_i|^2 + eps^2 + |d_j|^2)
+            // - 2 d_i.d_j and sum_j s3 (d_j - d_i) = M - d_i S over the
+            // sub-tile, 11 VALU + rsq per pair.  With |d_j| <= |d_i| / 3 the
+            // expansion loses at most a factor (4/2)^2 of relative accuracy
+            // in r^2 to cancellation, and M - d_i S none (|M| <= |d_i| S / 3).
+            const float ci = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+            const float bx = -2.f * dx, by = -2.f * dy, bz = -2.f * dz;
+            float sa = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
+#pragma unroll 16
+            for (int j = 0; j < kTile; ++j) {
+              const F4 pj = s_rel[j0 + j];
+              const float d2 =
+                  __builtin_fmaf(pj.x, bx, __builtin_fmaf(pj.y, by, __builtin_fmaf(pj.z, bz, ci + pj.w)));
+              const float inv = __builtin_amdgcn_rsqf(d2);
+              const float s3 = inv * (inv * inv);
+              sa += s3;
+              mx = __builtin_fmaf(pj.x, s3, mx);
+              my = __builtin_fmaf(pj.y, s3, my);
+              mz = __builtin_fmaf(pj.z, s3, mz);
+              a[3].sub(inv);
+            }
+            a[0].part += __builtin_fmaf(-dx, sa, mx);
+            a[1].part += __builtin_fmaf(-dy, sa, my);
+            a[2].part += __builtin_fmaf(-dz, sa, mz);
+          } else if (tile_form) {
 #pragma unroll 16
             for (int j = 0; j < kTile; ++j) unit_tile(j0 + j);
           } else {

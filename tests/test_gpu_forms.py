@@ -163,22 +163,27 @@ def test_mixed_form_passes_against_fp64_oracle(cs, orc, cfg, ph, grav_mode):
 
 
 def test_mixed_gravity_forms_agree(cs):
-    """The tile-relative and split forms of the mixed gravity kernel give the
-    same accelerations to FP32 rounding level on Peano-Hilbert-ordered input."""
+    """The tile-relative (expanded equal-mass body, and the body on r), and
+    split forms of the mixed gravity kernel give the same accelerations to
+    FP32 rounding level on Peano-Hilbert-ordered input."""
     n = 65536
     out = {}
-    for split in (0, 1):
+    for name, split, expand in (("expanded", 0, 1), ("tile", 0, 0), ("split", 1, 1)):
         e = cs.make_engine("mixed", 3, n, 7, ph_sort=True)
         e.set_scalar("grav_split_only", split)
+        e.set_scalar("grav_expand", expand)
         e.run_pass(cs.GRAVITY)
-        out[split] = e.get("grav")[:n].astype(np.float64)
+        out[name] = e.get("grav")[:n].astype(np.float64)
         e.close()
-    a, b = out[0][:, :3], out[1][:, :3]
+    b = out["split"][:, :3]
     rms = np.sqrt((b ** 2).sum(1).mean())
-    err = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-3 * rms)
-    assert err.max() < 2e-6, err.max()
-    assert not np.array_equal(a, b), "the tile-relative form was not taken"
-    np.testing.assert_allclose(out[0][:, 3], out[1][:, 3], rtol=2e-6)
+    for name in ("expanded", "tile"):
+        a = out[name][:, :3]
+        err = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-3 * rms)
+        assert err.max() < 2e-6, (name, err.max())
+        assert not np.array_equal(a, b), f"the {name} form was not taken"
+        np.testing.assert_allclose(out[name][:, 3], out["split"][:, 3], rtol=2e-6)
+    assert not np.array_equal(out["expanded"], out["tile"]), "the expanded body was not taken"
 
 
 @pytest.mark.parametrize("n", [100, 300, 1000])
