@@ -58,9 +58,12 @@ def run_form(cs, form, n, seed, warmup, steps, timing=True, i_order=0):
     st = e.stats()
     kbar = float(e.get("ngb_count")[:n].mean())
     import numpy as np
+    # the list pass leaves each lane's count of visited tree nodes in the hash
+    # keys (dev_find_ngb.cu:84-85)
+    walks = float(e.get("hash")[:, 0].astype(np.int64).sum())
     v = (e.get("vel64") if form == "mixed" else e.get("vel"))[:n].astype(np.float64)
     mom = (v[:, :3] * v[:, 3:4]).sum(0)
-    out = {"wall_s": wall, "kbar": kbar, "energies": energies, "timers": timers, "dt": e.scalar("dt"),
+    out = {"wall_s": wall, "kbar": kbar, "walks": walks, "energies": energies, "timers": timers, "dt": e.scalar("dt"),
            "momentum": mom, "momentum_scale": float((np.linalg.norm(v[:, :3], axis=1) * v[:, 3]).sum()),
            "angular_momentum": st["J"]}
     e.close()
@@ -80,7 +83,7 @@ SPH_BYTES = {
 }
 
 
-def sph_bandwidth(timers, steps, n, kbar):
+def sph_bandwidth(timers, steps, n, kbar, walks=None):
     out = {}
     for name, f in SPH_BYTES.items():
         if name not in timers:
@@ -90,6 +93,10 @@ def sph_bandwidth(timers, steps, n, kbar):
         gbps = f(kbar) * n / per_launch_s / 1e9
         out[name] = {"bytes_per_particle": f(kbar), "ms": per_launch_s * 1e3, "GBps": gbps,
                      "frac_of_8TBps": gbps / 8000.0}
+        if name == "find_ngb" and walks:
+            # the walk is latency-bound: tree nodes visited per second
+            out[name]["node_visits_per_particle"] = walks / n
+            out[name]["node_visits_per_s"] = walks / per_launch_s
     return out
 
 
@@ -203,6 +210,7 @@ SCALE_WORKLOAD = "uniform-density sphere (R=1, M=1) with a Gaussian k^-4 velocit
 def scale_leg(cs, n, steps, seed):
     """The scale workload: `steps` mixed-form steps (SPH GB/s, gravity FP32
     fraction) and one reverse-mode step of the FP64 form at the same N."""
+    import numpy as np
     e = cs.make_engine("mixed", 5, n, seed, ph_sort=True, i_order=0)
     e.set_scalar("timing", 1)
     e.init()
@@ -214,6 +222,7 @@ def scale_leg(cs, n, steps, seed):
     wall = time.perf_counter() - t0
     timers = e.timers()
     kbar = float(e.get("ngb_count")[:n].mean())
+    walks = float(e.get("hash")[:, 0].astype(np.int64).sum())
     nd = e.nd
     e.close()
     g_ms, g_calls = timers.get("gravity", (float("nan"), 1))
@@ -222,7 +231,7 @@ def scale_leg(cs, n, steps, seed):
     out = {"workload": SCALE_WORKLOAD, "n": n, "steps": steps, "form": "mixed",
            "value": n * steps / wall, "unit": "particle-updates/s", "ms_per_step": wall / steps * 1e3,
            "kernel_ms_per_step": {k: v[0] / steps for k, v in sorted(timers.items())},
-           "sph_hbm": {"mean_list_length": kbar, "passes": sph_bandwidth(timers, steps, n, kbar)},
+           "sph_hbm": {"mean_list_length": kbar, "passes": sph_bandwidth(timers, steps, n, kbar, walks)},
            "gravity": {"kernel": "k_gravity_mixed_plummer", "launch_ms": g_avg * 1e3, "achieved": achieved,
                        "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS}}
     print(json.dumps({"scale_progress": "mixed steps done", "ms_per_step": out["ms_per_step"]}), flush=True)
@@ -340,7 +349,7 @@ def main():
             "mixed": mom_drift(mixed), "plain": mom_drift(plain)},
         "kernel_ms_per_step": {k: v[0] / max(K, 1) for k, v in sorted(mixed["timers"].items())},
         "sph_hbm": {"mean_list_length": mixed["kbar"],
-                    "passes": sph_bandwidth(mixed["timers"], K, n, mixed["kbar"])},
+                    "passes": sph_bandwidth(mixed["timers"], K, n, mixed["kbar"], mixed["walks"])},
         "roofline": {"bound": "valu", "kernel": "k_gravity_mixed_plummer",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profile("k_gravity"),
