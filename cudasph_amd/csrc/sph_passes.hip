@@ -174,11 +174,36 @@ __global__ __launch_bounds__(kBlock) void k_find_ngb(const V4<T>* __restrict__ p
   const V4<T> q = pos[i];
   const int off = offset[i];
   const int cap = n_norm(cap_count[i], 4);
+  // ids go out four at a time as one 16-byte store when the slot run is
+  // 16-byte aligned (offsets are multiples of 4 in the reference layout): a
+  // wavefront's lanes write 64 different lists, so a 4-byte store per id
+  // would dirty a cache line per lane per id
+  const bool vec = (off & 3) == 0 && (reinterpret_cast<uintptr_t>(list) & 15) == 0;
   int c = 0;
+  int b0 = 0, b1 = 0, b2 = 0;   // the current group's first three ids
   const int nw = kd_walk<T>(q, tree, wp.n_nodes, wp.fix_phantom, s_stack + threadIdx.x, [&](T, int id) {
-    if (c < cap) list[off + c] = id;
+    const int r = c & 3;
+    if (vec) {
+      if (r == 3) {
+        if (c < cap) *reinterpret_cast<int4*>(list + off + c - 3) = make_int4(b0, b1, b2, id);
+      } else {
+        b0 = r == 0 ? id : b0;
+        b1 = r == 1 ? id : b1;
+        b2 = r == 2 ? id : b2;
+      }
+    } else if (c < cap) {
+      list[off + c] = id;
+    }
     ++c;
   });
+  if (vec) {   // the last, partial group (slots past the count stay untouched)
+    const int g = c & ~3, r = c & 3;
+    if (r > 0 && g < cap) {
+      list[off + g] = b0;
+      if (r > 1) list[off + g + 1] = b1;
+      if (r > 2) list[off + g + 2] = b2;
+    }
+  }
   hash[t].key = (uint32_t)nw;
   if (c > cap) {
     atomicOr(flags, 1);
