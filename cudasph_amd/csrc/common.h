@@ -39,6 +39,18 @@ struct HipError : std::runtime_error {
 __device__ __forceinline__ int node_bits(float w) { return __float_as_int(w); }
 __device__ __forceinline__ int node_bits(double w) { return (int)w; }
 
+// Workgroups are handed to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8, each XCD with its own L2).  xcd_block renumbers them so that XCD p
+// takes the p-th contiguous share of the grid: particles that are neighbours
+// in memory (Peano-Hilbert) order then run on the same XCD, and their
+// neighbour gathers and tree walks share its L2 instead of fetching the same
+// lines into all eight.  A bijection on [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int q = nb / kXcd, r = nb % kXcd, x = b % kXcd;
+  return x * q + (x < r ? x : r) + b / kXcd;
+}
+
 // n_norm (CUDAsph.h:16-20): round up to a multiple of j, at least j
 __host__ __device__ __forceinline__ int n_norm(int n, int j) {
   n = ((n - 1) / j) * j + j;

@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_reaction(const V4<T>* __restrict__ p
                                                       const T* __restrict__ ngb_acc, const KV* __restrict__ pairs,
                                                       const int* __restrict__ owner, const int* __restrict__ start,
                                                       const int* __restrict__ end, int n) {
-  const int j = blockIdx.x * kBlock + threadIdx.x;
+  const int j = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (j >= n) return;
   const V4<T> pj = pos[j];
   V4<T> d = bdots[j];
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_reaction_mixed(const F4* __restrict_
                                                             const int* __restrict__ start,
                                                             const int* __restrict__ end, int n) {
   constexpr int kG = 8;
-  const int g = blockIdx.x * kBlock + threadIdx.x;
+  const int g = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const int j = g / kG, lane = g % kG;
   if (j >= n) return;   // whole groups leave together
   const F4 pj = pos[j], lj = lo[j];

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_range(V4<T>* __restrict__ pos,
                                                          const V4<T>* __restrict__ tree, int n_dev,
                                                          WalkParams wp) {
   __shared__ int s_stack[kStackDepth * kBlock];
-  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
   const V4<T> q = pos[i];
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_find_ngb(const V4<T>* __restrict__ p
                                                       int* __restrict__ list, int* __restrict__ flags,
                                                       int n_dev, WalkParams wp, int clamp) {
   __shared__ int s_stack[kStackDepth * kBlock];
-  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
   const V4<T> q = pos[i];
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const V4<T>* __restrict__ po
                                                      V4<T>* __restrict__ dots, const KV* __restrict__ hash,
                                                      const int* __restrict__ offset, const int* __restrict__ count,
                                                      const int* __restrict__ list, int n_dev) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
   const V4<T> bp = pos[i], bv = vel[i];
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_hydro(const V4<T>* __restrict__ pos,
                                                    const KV* __restrict__ hash, const int* __restrict__ offset,
                                                    const int* __restrict__ count, const int* __restrict__ list,
                                                    int n_dev, T alpha, T beta) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
   const V4<T> bp = pos[i], bv = vel[i], hd = hydro[i], dd = dots[i];
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_density_mixed(const F4* __restrict__
                                                           const int* __restrict__ offset,
                                                           const int* __restrict__ count,
                                                           const int* __restrict__ list, int n_dev) {
-  const int g = blockIdx.x * kBlock + threadIdx.x;
+  const int g = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const int t = g / kGroup, lane = g % kGroup;
   if (t >= n_dev) return;   // whole groups leave together
   const int i = (int)hash[t].value;
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_hydro_mixed(const F4* __restrict__ p
                                                         const KV* __restrict__ hash, const int* __restrict__ offset,
                                                         const int* __restrict__ count, const int* __restrict__ list,
                                                         int n_dev, float alpha, float beta) {
-  const int gl = blockIdx.x * kBlock + threadIdx.x;
+  const int gl = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const int t = gl / kGroup, lane = gl % kGroup;
   if (t >= n_dev) return;   // whole groups leave together
   const int i = (int)hash[t].value;
