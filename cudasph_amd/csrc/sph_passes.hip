@@ -26,12 +26,15 @@ namespace cudasph {
 // what the reference's out-of-range texture fetch returns.  fix_phantom drops
 // that push instead.  Returns the number of nodes visited.
 template <class T, class Visit>
-__device__ __forceinline__ int kd_walk(const V4<T>& q, const V4<T>* __restrict__ tree, int n_nodes,
+__device__ __forceinline__ int kd_walk(const V4<T> q, const V4<T>* __restrict__ tree, int n_nodes,
                                        int fix_phantom, int* __restrict__ stk, Visit&& visit) {
   int top = 0;
   stk[0] = 1;
   int n_walk = 0;
-  const T r2 = q.w * q.w;
+  // the query by value, in scalars: selecting a component of a reference
+  // becomes a select of addresses, which keeps the caller's copy in scratch
+  const T qx = q.x, qy = q.y, qz = q.z, qw = q.w;
+  const T r2 = qw * qw;
   while (top >= 0) {
     ++n_walk;
     int id = stk[top * kBlock];
@@ -40,13 +43,13 @@ __device__ __forceinline__ int kd_walk(const V4<T>& q, const V4<T>* __restrict__
     const int split = (bits >> 24) & 3;
     id <<= 1;
     if (id <= n_nodes && split < 3) {
-      const T qc = split == 0 ? q.x : (split == 1 ? q.y : q.z);
-      const T nc = split == 0 ? node.x : (split == 1 ? node.y : node.z);
-      if (qc - q.w < nc) stk[(top++) * kBlock] = id;
-      if (qc + q.w > nc && (!fix_phantom || id < n_nodes)) stk[(top++) * kBlock] = id + 1;
+      const T qc = (split & 2) ? qz : ((split & 1) ? qy : qx);
+      const T nc = (split & 2) ? node.z : ((split & 1) ? node.y : node.x);
+      if (qc - qw < nc) stk[(top++) * kBlock] = id;
+      if (qc + qw > nc && (!fix_phantom || id < n_nodes)) stk[(top++) * kBlock] = id + 1;
     }
     --top;
-    const T dx = q.x - node.x, dy = q.y - node.y, dz = q.z - node.z;
+    const T dx = qx - node.x, dy = qy - node.y, dz = qz - node.z;
     const T ds2 = dx * dx + dy * dy + dz * dz;
     if (ds2 < r2) visit(ds2, bits & ~(3 << 24));
   }
