@@ -13,7 +13,16 @@
 namespace cudasph {
 
 constexpr int kBlock = 256;        // threads per block for per-particle kernels
-constexpr int kStackDepth = 28;    // kd-walk stack entries per lane: tree height + 1 <= 25 for n < 2^24
+// kd-walk stack entries per lane.  Popping a node at depth d leaves at most
+// one pending sibling per level 1..d on the stack and pushes at most two
+// children, so a tree of H levels (H = floor(log2 n) + 1) needs at most H
+// entries; one more is kept spare.  Sizing the LDS stack to the tree
+// (22 entries at n = 2^20 instead of a fixed 28) lets more blocks share a CU.
+__host__ __device__ __forceinline__ int walk_stack_depth(int n_nodes) {
+  int h = 1;
+  while (h < 31 && (n_nodes >> h) > 0) ++h;   // h = floor(log2 n) + 1 for n >= 1
+  return h + 1;
+}
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 

@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_range(V4<T>* __restrict__ pos,
                                                          const KV* __restrict__ hash,
                                                          const V4<T>* __restrict__ tree, int n_dev,
                                                          WalkParams wp) {
-  __shared__ int s_stack[kStackDepth * kBlock];
+  extern __shared__ int s_stack[];   // walk_stack_depth(n_nodes) entries per lane
   const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_find_ngb(const V4<T>* __restrict__ p
                                                       const int* cap_count, int* count_out,
                                                       int* __restrict__ list, int* __restrict__ flags,
                                                       int n_dev, WalkParams wp, int clamp) {
-  __shared__ int s_stack[kStackDepth * kBlock];
+  extern __shared__ int s_stack[];   // walk_stack_depth(n_nodes) entries per lane
   const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
@@ -544,7 +544,8 @@ __global__ __launch_bounds__(kBlock) void k_hydro_mixed(const F4* __restrict__ p
 template <class T>
 void launch_solve_range(V4<T>* pos, int* n_ngb, const KV* hash, const V4<T>* tree, int n_dev,
                         const WalkParams& wp, hipStream_t s) {
-  hipLaunchKernelGGL(k_solve_range<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, n_ngb, hash,
+  const size_t lds = (size_t)walk_stack_depth(wp.n_nodes) * kBlock * sizeof(int);
+  hipLaunchKernelGGL(k_solve_range<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), lds, s, pos, n_ngb, hash,
                      tree, n_dev, wp);
   CS_CHECK_LAUNCH();
 }
@@ -553,7 +554,8 @@ template <class T>
 void launch_find_ngb(const V4<T>* pos, const V4<T>* tree, KV* hash, const int* offset, const int* cap_count,
                      int* count_out, int* list, int* flags, int n_dev, const WalkParams& wp, int clamp,
                      hipStream_t s) {
-  hipLaunchKernelGGL(k_find_ngb<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), 0, s, pos, tree, hash, offset,
+  const size_t lds = (size_t)walk_stack_depth(wp.n_nodes) * kBlock * sizeof(int);
+  hipLaunchKernelGGL(k_find_ngb<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), lds, s, pos, tree, hash, offset,
                      cap_count, count_out, list, flags, n_dev, wp, clamp);
   CS_CHECK_LAUNCH();
 }
