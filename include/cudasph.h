@@ -168,6 +168,17 @@ void   sph_engine_destroy(void* e);
  * (n_dev+1), ngb_list, ngb_acc.  Returns bytes copied or < 0. */
 long   sph_engine_set(void* e, const char* name, const void* host, long n_bytes);
 long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
+/* Scalars.  Read and write: dt, global_time, iteration; exact_rsqrt (plain
+ * form: correctly rounded 1/sqrt in gravity, the CPU restatement's bits; 0
+ * is the hardware rsq the reference uses); i_order (per-particle kernels in
+ * memory order 0 or kd-tree heap order 1); overlap (gravity on a second
+ * stream beside the SPH passes); timing (per-kernel HIP-event timers).
+ * Mixed-form gravity measurement switches, all results within the form's
+ * tolerance: grav_split_only (1: split hi/lo differences for every pair),
+ * grav_unit_mass (0: the general-mass body even when every mass is equal),
+ * grav_expand (0: the equal-mass tile body on r instead of the expanded one).
+ * Read only: ngb_tot, overflow_reruns, cap, real_bytes, form.  Unknown names
+ * return NaN (get) or -1 with sph_last_error() (set). */
 double sph_engine_get_scalar(void* e, const char* name);
 int    sph_engine_set_scalar(void* e, const char* name, double v);
 int    sph_engine_pass(void* e, int pass);
