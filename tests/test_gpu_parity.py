@@ -153,3 +153,37 @@ def test_fp64_form_matches_fp64_oracle(cs, orc, cfg, n):
     for name in ("pos", "vel", "hydro", "dots", "bdots", "grav"):
         a, b = e.get(name)[:n], o.get(name)[:n]
         np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-12 * np.abs(b).max(), err_msg=name)
+
+
+def test_list_slot_overflow_recount_bit_exact(cs, orc):
+    """A particle that finds more neighbours than its list slot holds makes
+    the list pass rebuild the offsets from the true counts and walk again
+    (the reference's h-solve count can fall short of the count at the final
+    range).  Forced here by setting every n_ngb to 1 (4-slot runs) after the
+    h-solve: engine and oracle each rerun once, and the lists, counts and
+    offsets agree slot for slot."""
+    n = 16384
+    e, o, _ = make_pair(cs, orc, 2, n)
+    o.init()
+    copy_state(e, o)
+    e.run_pass(cs.TREE)
+    o.run_pass(orc.TREE)
+    e.run_pass(cs.SOLVE)
+    o.run_pass(orc.SOLVE)
+    short = np.ones_like(e.get("n_ngb"))
+    e.set("n_ngb", short)
+    o.set("n_ngb", short)
+    e.run_pass(cs.OFFSETS)
+    o.run_pass(orc.OFFSETS)
+    r_e, r_o = e.scalar("overflow_reruns"), o.scalar("overflow_reruns")
+    e.run_pass(cs.FIND)
+    o.run_pass(orc.FIND)
+    assert e.scalar("overflow_reruns") == r_e + 1
+    assert o.scalar("overflow_reruns") == r_o + 1
+    cnt = e.get("ngb_count")
+    assert np.array_equal(cnt, o.get("ngb_count"))
+    assert (cnt[:n] > 4).mean() > 0.99, "the short slots did not overflow"
+    assert np.array_equal(e.get("ngb_offset"), o.get("ngb_offset"))
+    le, lo = lists_of(e, n), lists_of(o, n)
+    bad = [i for i in range(n) if not np.array_equal(le[i], lo[i])]
+    assert not bad, f"{len(bad)} neighbour lists differ, first particle {bad[0]}"
