@@ -105,6 +105,65 @@ def test_solve_range_and_sph_accelerations_drop_in(cs, orc, preset):
         d.free()
 
 
+def test_sph_accelerations_truncates_an_outgrown_slot(cs, orc, capfd):
+    """Through the reference ABI the caller owns the offsets, so a list that
+    outgrows its slot cannot be re-laid out: it is truncated to the slot, the
+    actual length goes back into n_ngb and a message is printed (the reference
+    would overwrite the next particle's slot).  Forced by shrinking every
+    n_ngb to 1 (4-slot runs) after host_solve_range."""
+    n = 16384
+    p = cs.preset_params(1, n)
+    L = cs.lib()
+    L.sph_abi_set_params(C.byref(p), 1)
+    pos, vel, hyd = cs.generate_ic(1, n, 21)
+    o = orc.Oracle(32, p)
+    o.load_ic(pos, vel, hyd)
+    o.run_pass(orc.TREE)
+    nd = p.n_dev
+    d = Dev(L)
+    try:
+        hash_ = np.zeros((nd, 2), np.uint32)
+        hash_[:, 0] = np.arange(nd)
+        hash_[:, 1] = np.arange(nd)
+        dpos, dvel = d.put(o.get("pos")), d.put(o.get("vel"))
+        dtree, dhash = d.put(o.get("tree")[: nd + 1]), d.put(hash_)
+        dmap = d.put(np.zeros(nd, np.int32))
+        dn = d.put(np.zeros(nd, np.int32))
+        L.host_solve_range.argtypes = [C.c_int, C.c_int, C.c_float] + [C.c_void_p] * 6 + [C.c_int]
+        L.host_solve_range.restype = C.c_float
+        L.host_solve_range(nd, n, p.rel_err, dhash, dmap, dpos, dvel, dtree, dn, 0)
+        ones = np.ones(nd, np.int32)
+        L.copyArrayToDevice(dn, ones.ctypes.data, ones.nbytes, 0)
+        off = (4 * np.arange(nd + 1)).astype(np.int32)
+        dlist, dacc = d.put(np.full(4 * nd, -7, np.int32)), d.put(np.zeros(4 * nd, np.float32))
+        doff = d.put(off[:nd])
+        dhyd, ddots = d.put(o.get("hydro")), d.put(np.zeros((nd, 4), np.float32))
+        dbd, dh2 = d.put(np.zeros((nd, 4), np.float32)), d.put(np.zeros((nd, 4), np.float32))
+        L.host_sph_accelerations.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float] + [C.c_void_p] * 10 + [C.c_int]
+        L.host_sph_accelerations.restype = C.c_float
+        capfd.readouterr()
+        L.host_sph_accelerations(nd, 4 * nd, p.av_alpha, p.av_beta, dhash, None, dmap, dlist, doff, dhyd, ddots,
+                                 dbd, dh2, dacc, 0)
+        err = capfd.readouterr().err
+        assert "truncated" in err
+        # the oracle's untruncated lists at the same ranges (its h-solve from
+        # the same start gives the ABI's ranges bit for bit)
+        for pid in (orc.SOLVE, orc.OFFSETS, orc.FIND):
+            o.run_pass(pid)
+        assert_bits(d.get(dpos, o.get("pos")), o.get("pos"), "range")
+        ocnt, ooff, olst = o.get("ngb_count"), o.get("ngb_offset"), o.get("ngb_list")
+        cnt = d.get(dn, ones)
+        assert np.array_equal(cnt[:n], np.minimum(ocnt[:n], 4))
+        assert (ocnt[:n] > 4).mean() > 0.99, "the slots did not overflow"
+        lst = d.get(dlist, np.zeros(4 * nd, np.int32))
+        for i in range(0, n, 37):
+            k = int(cnt[i])
+            assert np.array_equal(lst[4 * i: 4 * i + k], olst[ooff[i]: ooff[i] + k]), i
+        assert np.isfinite(d.get(dbd, o.get("bdots"))[:n]).all()
+    finally:
+        d.free()
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_compute_cuda_gravity_drop_in(cs, orc, mode):
     n = 8192
