@@ -157,8 +157,14 @@ __device__ __forceinline__ void pair_plummer(A* a, T rx, T ry, T rz, T mj, T eps
 
 constexpr int kGravBlock = 256;
 constexpr int kTile = 64;        // j per blocked-compensation partial (mixed form)
-constexpr int kJTile = 512;      // j records staged in LDS per block iteration
+constexpr int kJTile = 512;      // j records staged in LDS per block iteration (reference-layout kernels)
 constexpr int kGravParts = 8;    // j-parts of the mixed Plummer kernel, one per XCD
+// j records per LDS tile of the mixed Plummer kernel: one 64-record sub-tile
+// per wavefront.  Its 12 KB of LDS lets 8 blocks share a CU (the kernel's 63
+// VGPRs allow 8 waves per SIMD), so the 4096 x 8 blocks at N = 2^20 run in
+// exactly 16 rounds; 512-record tiles (24 KB) fit 6 blocks and left the last
+// of 21 1/3 rounds a third full (measured 2 % slower)
+constexpr int kJTileMixed = kGravBlock;
 
 // Block-cooperative staging of j records into LDS.  Every lane of a wave then
 // reads the same LDS address (a broadcast read), so one j record serves the
@@ -239,6 +245,7 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
 // streams the whole j-set and re-fetches it from HBM.  Each block writes its
 // compensated (hi, lo) partial sums; k_gravity_combine adds the parts in
 // order.
+template <int JT>
 __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* __restrict__ hi,
                                                                        const F4* __restrict__ lo,
                                                                        F4* __restrict__ part_hi,
@@ -253,11 +260,11 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
   const bool uniform = (split_only & 2) == 0 && mass_range[0] == mass_range[1];
   const bool expand = uniform && (split_only & 4) == 0;
   split_only &= 1;
-  constexpr int kSub = kJTile / kTile;   // sub-tiles of kTile records, each with its own origin
-  __shared__ F4 s_hi[kJTile], s_lo[kJTile], s_rel[kJTile];
+  constexpr int kSub = JT / kTile;   // sub-tiles of kTile records, each with its own origin
+  __shared__ F4 s_hi[JT], s_lo[JT], s_rel[JT];
   __shared__ F4 s_c[kSub];
   __shared__ float s_r2[kSub];
-  static_assert(kTile == 64 && kJTile == 2 * kGravBlock, "each wavefront stages two whole sub-tiles");
+  static_assert(kTile == 64 && JT % kGravBlock == 0, "each wavefront stages whole sub-tiles");
   const int part = blockIdx.x % kGravParts;
   const int i = (blockIdx.x / kGravParts) * kGravBlock + threadIdx.x;
   const int ii = i < n_i ? i : n_i - 1;
@@ -265,12 +272,12 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
   const F4 pi = hi[ii], li = lo[ii];
   GAcc<float, 1> a[4];
   const int j_begin = min(part * j_chunk, n_j), j_end = min(j_begin + j_chunk, n_j);
-  for (int t0 = j_begin; t0 < j_end; t0 += kJTile) {
-    const int cnt = min(kJTile, j_end - t0);
+  for (int t0 = j_begin; t0 < j_end; t0 += JT) {
+    const int cnt = min(JT, j_end - t0);
     __syncthreads();
-    // stage: this wavefront's two sub-tiles, each relative to its middle
+    // stage: this wavefront's sub-tiles, each relative to its middle
     // record, and each sub-tile's squared radius about that record
-    for (int q = threadIdx.x; q < kJTile; q += kGravBlock) {
+    for (int q = threadIdx.x; q < JT; q += kGravBlock) {
       const int sub = q / kTile, sub0 = sub * kTile;
       float r2 = 0.f;
       if (sub0 < cnt) {
@@ -543,8 +550,8 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
     const int chunk = ceil_div(ceil_div(n_j, kGravParts), kTile) * kTile;
     F4* part_hi = part;
     F4* part_lo = part + (size_t)kGravParts * n_i;
-    hipLaunchKernelGGL(k_gravity_mixed_plummer, dim3(ceil_div(n_i, kGravBlock) * kGravParts), block, 0, s, hi, lo,
-                       part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range);
+    hipLaunchKernelGGL(k_gravity_mixed_plummer<kJTileMixed>, dim3(ceil_div(n_i, kGravBlock) * kGravParts), block, 0,
+                       s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range);
     CS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gravity_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part_hi, part_lo, out,
                        n_i, mass_range, split_only);
