@@ -122,6 +122,7 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
     gvel32_.alloc(nd_);
     gpart_.alloc(2 * 8 * (size_t)nd_);
     gmass_.alloc(2 + 2 * (size_t)ceil_div(n_, kBlock));
+    gmom_.alloc(52 * (size_t)ceil_div(nd_, 64));
     CS_HIP(hipMemsetAsync(pos_lo_, 0, pos_lo_.bytes(), s_));
     CS_HIP(hipMemsetAsync(pos64_, 0, pos64_.bytes(), s_));
     CS_HIP(hipMemsetAsync(vel64_, 0, vel64_.bytes(), s_));
@@ -237,6 +238,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_split_only") return grav_split_only_;
   if (name == "grav_unit_mass") return grav_unit_mass_;
   if (name == "grav_expand") return grav_expand_;
+  if (name == "grav_far") return grav_far_;
   if (name == "overlap") return overlap_;
   if (name == "timing") return timers_.enabled;
   if (name == "cap") return (double)cap_;
@@ -265,6 +267,8 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     grav_unit_mass_ = (int)v;
   } else if (name == "grav_expand") {
     grav_expand_ = (int)v;
+  } else if (name == "grav_far") {
+    grav_far_ = (int)v;
   } else if (name == "overlap") {
     overlap_ = (int)v;
   } else if (name == "timing") {
@@ -533,8 +537,11 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
   TimerScope ts(timers_, "gravity", gs);
   if constexpr (kF32) {
     if (mixed()) {
-      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, gmass_, nd_, nd_, n_, p_.grav_mode,
-                           p_.grav_eps, grav_split_only_ | (grav_unit_mass_ ? 0 : 2) | (grav_expand_ ? 0 : 4), gs);
+      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, gmass_, gmom_, nd_, nd_, n_, p_.grav_mode,
+                           p_.grav_eps,
+                           grav_split_only_ | (grav_unit_mass_ ? 0 : 2) | (grav_expand_ ? 0 : 4) |
+                               (grav_far_ ? 16 : 0),
+                           gs);
       return;
     }
   }

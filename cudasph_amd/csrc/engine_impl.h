@@ -176,6 +176,7 @@ class Engine final : public EngineBase {
   DBuf<double> u64_;
   DBuf<F4> pos_lo_, ghi_, glo_, gvel32_, gpart_;
   DBuf<float> gmass_;   // {min, max} of the masses + reduction partials (mixed gravity)
+  DBuf<float> gmom_;    // per-sub-tile multipole records (mixed gravity far field)
   bool dirty64_ = false;
 
   HBuf<V4<T>> h_pos_, h_tree_;
@@ -191,6 +192,7 @@ class Engine final : public EngineBase {
   int grav_split_only_ = 0;
   int grav_unit_mass_ = 1;    // mixed gravity: unit-mass pair body when every mass is equal
   int grav_expand_ = 1;       // mixed gravity: expanded r^2 in the equal-mass tile body
+  int grav_far_ = 0;          // mixed gravity: order-4 multipole far field for distant sub-tiles
   int i_order_ = 1;   // 1: lanes take particles in kd-tree heap order, 0: in memory order
   long ngb_tot_ = 0;
   int overflow_reruns_ = 0;

@@ -212,6 +212,164 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
   if (i < n_i) out[i] = {a[0].get(), a[1].get(), a[2].get(), a[3].get()};
 }
 
+// ---------------------------------------------------------------------------
+// Multipole far field of the mixed Plummer kernel (optional, flags bit 4).
+//
+// Each 64-record sub-tile of the j-set (records [64 s, 64 s + 64), the
+// sub-tiles the kernel below stages) gets its moments up to order 4 about its
+// centre of mass, for the softened kernel phi(y) = (|y|^2 + eps^2)^(-1/2):
+//   sum_j m_j phi(y - e_j) = sum_n (-1)^n / n! M^(n) . D^(n)(y),
+//   M^(n) = sum_j m_j e_j^(x n),  D^(n) = grad^n phi,  e_j = x_j - COM.
+// phi depends on u = |y|^2 / 2 only, so D^(n) is a sum over pairings of its
+// indices: g_{n-m} times y^(n-2m) and m Kronecker deltas, with
+// g_k = d^k phi / du^k = (-1)^k (2k-1)!! s^-(2k+1), s^2 = |y|^2 + eps^2.
+// Contracted with the (not traceless: phi is not harmonic under softening)
+// moments this needs M2, M3, M4, their traces tr M2, t3 = tr M3 and
+// Q = tr M4, tr Q.  M^(1) = 0 about the centre of mass.  Truncation after
+// order 4 leaves a relative error ~ (R / |y|)^5 of the sub-tile's force
+// (R = max |e_j|): the far field is used for a (wavefront, sub-tile) only
+// when every lane has |y| >= kFarRatio R, where it is ~1e-7.
+constexpr int kMomWords = 52;     // floats per sub-tile record, see k_subtile_moments
+constexpr float kFarRatio = 16.f;
+
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// record of sub-tile s: [0..2] centre-of-mass offset from the sub-tile's
+// middle record (the tile form's centre), [3] R^2, [4] M0, [5] tr M2,
+// [6] tr Q, [8..13] M2 (xx xy xz yy yz zz), [14..16] t3, [17..22] Q,
+// [23..32] M3 (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz), [33..47] M4
+// (xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz xyzz xzzz yyyy yyyz yyzz yzzz
+// zzzz).  Masses as the kernel uses them (1 when every mass is equal);
+// padding records (j >= n_real) are left out.  One wavefront per sub-tile.
+__global__ __launch_bounds__(kGravBlock) void k_subtile_moments(const F4* __restrict__ hi, const F4* __restrict__ lo,
+                                                                float* __restrict__ mom, int n_j, int n_real,
+                                                                int n_sub, int flags,
+                                                                const float* __restrict__ mass_range) {
+  const bool uniform = (flags & 2) == 0 && mass_range[0] == mass_range[1];
+  const int s = blockIdx.x * (kGravBlock / 64) + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= n_sub) return;   // whole wavefronts
+  const int j0 = s * kTile, nj = min(kTile, n_j - j0);
+  const F4 c = hi[j0 + nj / 2];
+  const int j = j0 + lane;
+  const bool valid = lane < nj && j < n_real;
+  float dx = 0.f, dy = 0.f, dz = 0.f, m = 0.f;
+  if (valid) {
+    const F4 h = hi[j], l = lo[j];
+    dx = (h.x - c.x) + l.x;
+    dy = (h.y - c.y) + l.y;
+    dz = (h.z - c.z) + l.z;
+    m = uniform ? 1.f : h.w;
+  }
+  const float M0 = wave_sum(m);
+  const float sx = wave_sum(m * dx), sy = wave_sum(m * dy), sz = wave_sum(m * dz);
+  const float mx = M0 > 0.f ? sx / M0 : 0.f, my = M0 > 0.f ? sy / M0 : 0.f, mz = M0 > 0.f ? sz / M0 : 0.f;
+  const float ex = valid ? dx - mx : 0.f, ey = valid ? dy - my : 0.f, ez = valid ? dz - mz : 0.f;
+  float r2 = __builtin_fmaf(ex, ex, __builtin_fmaf(ey, ey, ez * ez));
+  for (int o = 32; o > 0; o >>= 1) r2 = fmaxf(r2, __shfl_xor(r2, o));
+  const float mex = m * ex, mey = m * ey, mez = m * ez;
+  const float xx = mex * ex, xy = mex * ey, xz = mex * ez, yy = mey * ey, yz = mey * ez, zz = mez * ez;
+  const float xxx = xx * ex, xxy = xx * ey, xxz = xx * ez, xyy = xy * ey, xyz = xy * ez, xzz = xz * ez,
+              yyy = yy * ey, yyz = yy * ez, yzz = yz * ez, zzz = zz * ez;
+  const float p4[15] = {xxx * ex, xxx * ey, xxx * ez, xxy * ey, xxy * ez, xxz * ez, xyy * ey, xyy * ez,
+                        xyz * ez, xzz * ez, yyy * ey, yyy * ez, yyz * ez, yzz * ez, zzz * ez};
+  const float p2[6] = {xx, xy, xz, yy, yz, zz};
+  const float p3[10] = {xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz};
+  float M2[6], M3[10], M4[15];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) M2[k] = wave_sum(p2[k]);
+#pragma unroll
+  for (int k = 0; k < 10; ++k) M3[k] = wave_sum(p3[k]);
+#pragma unroll
+  for (int k = 0; k < 15; ++k) M4[k] = wave_sum(p4[k]);
+  if (lane == 0) {
+    float* r = mom + (size_t)s * kMomWords;
+    const float Q[6] = {M4[0] + M4[3] + M4[5], M4[1] + M4[6] + M4[8], M4[2] + M4[7] + M4[9],
+                        M4[3] + M4[10] + M4[12], M4[4] + M4[11] + M4[13], M4[5] + M4[12] + M4[14]};
+    r[0] = mx;
+    r[1] = my;
+    r[2] = mz;
+    r[3] = r2;
+    r[4] = M0;
+    r[5] = M2[0] + M2[3] + M2[5];
+    r[6] = Q[0] + Q[3] + Q[5];
+    r[7] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r[8 + k] = M2[k];
+    r[14] = M3[0] + M3[3] + M3[5];
+    r[15] = M3[1] + M3[6] + M3[8];
+    r[16] = M3[2] + M3[7] + M3[9];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r[17 + k] = Q[k];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) r[23 + k] = M3[k];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) r[33 + k] = M4[k];
+    r[48] = r[49] = r[50] = r[51] = 0.f;
+  }
+}
+
+// acceleration G = sum_j m_j (x_j - x_i) s^-3 and potential -sum_j m_j / s of
+// one sub-tile from its record M, at y = x_i - COM (order-4 expansion)
+__device__ __forceinline__ void far_field(const float* __restrict__ M, float yx, float yy, float yz, float eps2,
+                                          float& gx, float& gy, float& gz, float& pot) {
+  const float s2 = __builtin_fmaf(yx, yx, __builtin_fmaf(yy, yy, __builtin_fmaf(yz, yz, eps2)));
+  const float inv = __builtin_amdgcn_rsqf(s2), i2 = inv * inv;
+  const float g0 = inv, g1 = -inv * i2, g2 = -3.f * g1 * i2, g3 = -5.f * g2 * i2, g4 = -7.f * g3 * i2,
+              g5 = -9.f * g4 * i2;
+  const float M0 = M[4], trM2 = M[5], trQ = M[6];
+  const float* m2 = M + 8;
+  const float* t3 = M + 14;
+  const float* Q = M + 17;
+  const float* m3 = M + 23;
+  const float* m4 = M + 33;
+  const float M2x = __builtin_fmaf(m2[0], yx, __builtin_fmaf(m2[1], yy, m2[2] * yz));
+  const float M2y = __builtin_fmaf(m2[1], yx, __builtin_fmaf(m2[3], yy, m2[4] * yz));
+  const float M2z = __builtin_fmaf(m2[2], yx, __builtin_fmaf(m2[4], yy, m2[5] * yz));
+  const float yM2y = __builtin_fmaf(M2x, yx, __builtin_fmaf(M2y, yy, M2z * yz));
+  const float Qx = __builtin_fmaf(Q[0], yx, __builtin_fmaf(Q[1], yy, Q[2] * yz));
+  const float Qy = __builtin_fmaf(Q[1], yx, __builtin_fmaf(Q[3], yy, Q[4] * yz));
+  const float Qz = __builtin_fmaf(Q[2], yx, __builtin_fmaf(Q[4], yy, Q[5] * yz));
+  const float yQy = __builtin_fmaf(Qx, yx, __builtin_fmaf(Qy, yy, Qz * yz));
+  const float t3y = __builtin_fmaf(t3[0], yx, __builtin_fmaf(t3[1], yy, t3[2] * yz));
+  // quadratic monomials, off-diagonal ones doubled: (M3 . y y)_k
+  const float qxx = yx * yx, qxy = 2.f * yx * yy, qxz = 2.f * yx * yz, qyy = yy * yy, qyz = 2.f * yy * yz,
+              qzz = yz * yz;
+  auto dot6 = [&](float a, float b, float c, float d, float e, float f) {
+    return __builtin_fmaf(qxx, a, __builtin_fmaf(qxy, b, __builtin_fmaf(qxz, c,
+           __builtin_fmaf(qyy, d, __builtin_fmaf(qyz, e, qzz * f)))));
+  };
+  const float M3x = dot6(m3[0], m3[1], m3[2], m3[3], m3[4], m3[5]);
+  const float M3y = dot6(m3[1], m3[3], m3[4], m3[6], m3[7], m3[8]);
+  const float M3z = dot6(m3[2], m3[4], m3[5], m3[7], m3[8], m3[9]);
+  const float T3 = __builtin_fmaf(M3x, yx, __builtin_fmaf(M3y, yy, M3z * yz));
+  // cubic monomials with their multinomial weights: (M4 . y y y)_k
+  const float cxxx = qxx * yx, cxxy = 3.f * qxx * yy, cxxz = 3.f * qxx * yz, cxyy = 3.f * yx * qyy,
+              cxyz = 3.f * qxy * yz, cxzz = 3.f * yx * qzz, cyyy = qyy * yy, cyyz = 3.f * qyy * yz,
+              cyzz = 3.f * yy * qzz, czzz = qzz * yz;
+  auto dot10 = [&](int a, int b, int c, int d, int e, int f, int g, int h, int k, int l) {
+    return __builtin_fmaf(cxxx, m4[a], __builtin_fmaf(cxxy, m4[b], __builtin_fmaf(cxxz, m4[c],
+           __builtin_fmaf(cxyy, m4[d], __builtin_fmaf(cxyz, m4[e], __builtin_fmaf(cxzz, m4[f],
+           __builtin_fmaf(cyyy, m4[g], __builtin_fmaf(cyyz, m4[h], __builtin_fmaf(cyzz, m4[k], czzz * m4[l])))))))));
+  };
+  const float M4x = dot10(0, 1, 2, 3, 4, 5, 6, 7, 8, 9);
+  const float M4y = dot10(1, 3, 4, 6, 7, 8, 10, 11, 12, 13);
+  const float M4z = dot10(2, 4, 5, 7, 8, 9, 11, 12, 13, 14);
+  const float T4 = __builtin_fmaf(M4x, yx, __builtin_fmaf(M4y, yy, M4z * yz));
+  const float S = M0 * g0 + 0.5f * (g2 * yM2y + g1 * trM2) - (g3 * T3 + 3.f * g2 * t3y) * (1.f / 6.f) +
+                  (g4 * T4 + 6.f * g3 * yQy + 3.f * g2 * trQ) * (1.f / 24.f);
+  const float A = M0 * g1 + 0.5f * (g3 * yM2y + g2 * trM2) - (g4 * T3 + 3.f * g3 * t3y) * (1.f / 6.f) +
+                  (g5 * T4 + 6.f * g4 * yQy + 3.f * g3 * trQ) * (1.f / 24.f);
+  const float b2 = g2, b3 = -0.5f * g3, bt = -0.5f * g2, b4 = g4 * (1.f / 6.f), bq = 0.5f * g3;
+  gx = A * yx + b2 * M2x + b3 * M3x + bt * t3[0] + b4 * M4x + bq * Qx;
+  gy = A * yy + b2 * M2y + b3 * M3y + bt * t3[1] + b4 * M4y + bq * Qy;
+  gz = A * yz + b2 * M2z + b3 * M3z + bt * t3[2] + b4 * M4z + bq * Qz;
+  pot = -S;
+}
+
 // mixed form, Plummer: split positions, FMA pair arithmetic, blocked
 // compensated sums.  hi = (x,y,z,m) rounded, lo = (x-hi, y-hi, z-hi, 0).
 //
@@ -245,25 +403,30 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
 // streams the whole j-set and re-fetches it from HBM.  Each block writes its
 // compensated (hi, lo) partial sums; k_gravity_combine adds the parts in
 // order.
-template <int JT>
+template <int JT, bool FAR>
 __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* __restrict__ hi,
                                                                        const F4* __restrict__ lo,
                                                                        F4* __restrict__ part_hi,
                                                                        F4* __restrict__ part_lo, int n_i,
                                                                        int n_j, int j_chunk, float eps2,
                                                                        int split_only,
-                                                                       const float* __restrict__ mass_range) {
+                                                                       const float* __restrict__ mass_range,
+                                                                       const float* __restrict__ mom) {
   // every mass equal (mass_range = {min, max} over the real particles): the
   // pair body drops its mass factor and k_gravity_combine scales by m once;
   // padding records sit at 1e30, where r^2 overflows and rsq returns 0, so
   // they add exactly nothing without their zero mass
   const bool uniform = (split_only & 2) == 0 && mass_range[0] == mass_range[1];
   const bool expand = uniform && (split_only & 4) == 0;
+  // a separate instantiation: the far-field code raises the VGPR count, which
+  // would lower the occupancy of the pairwise kernel
+  const bool far_on = FAR && (split_only & 1) == 0;
   split_only &= 1;
   constexpr int kSub = JT / kTile;   // sub-tiles of kTile records, each with its own origin
   __shared__ F4 s_hi[JT], s_lo[JT], s_rel[JT];
   __shared__ F4 s_c[kSub];
   __shared__ float s_r2[kSub];
+  __shared__ __align__(16) float s_mom[kSub][FAR ? kMomWords : 1];
   static_assert(kTile == 64 && JT % kGravBlock == 0, "each wavefront stages whole sub-tiles");
   const int part = blockIdx.x % kGravParts;
   const int i = (blockIdx.x / kGravParts) * kGravBlock + threadIdx.x;
@@ -298,6 +461,12 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
         }
       }
     }
+    if constexpr (FAR) {   // the tile's sub-tile records (t0 is a multiple of kTile)
+      for (int q = threadIdx.x; q < kSub * kMomWords; q += kGravBlock) {
+        const int sub = q / kMomWords, w = q % kMomWords;
+        if (sub * kTile < cnt) s_mom[sub][w] = mom[(size_t)(t0 / kTile + sub) * kMomWords + w];
+      }
+    }
     __syncthreads();
     auto pair = [&](float rx, float ry, float rz, float mj) {
       const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
@@ -325,6 +494,24 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
       // tile form only if every lane of the wavefront has |d_i| >= 3 R_sub
       // (uniform per wavefront: no divergence); the squared test carries a
       // margin of 1.1 for the rounding of the squares
+      if (FAR && far_on) {
+        // multipole far field when every lane is kFarRatio sub-tile radii
+        // from its centre of mass (uniform per wavefront)
+        const float* M = &s_mom[sub][0];
+        const float yx = dx - M[0], yy = dy - M[1], yz = dz - M[2];
+        const float y2 = __builtin_fmaf(yx, yx, __builtin_fmaf(yy, yy, yz * yz));
+        if (__all(y2 >= kFarRatio * kFarRatio * M[3])) {
+          float gx, gy, gz, pt;
+          far_field(M, yx, yy, yz, eps2, gx, gy, gz, pt);
+          a[0].part += gx;
+          a[1].part += gy;
+          a[2].part += gz;
+          a[3].part += pt;
+          if ((sub & 1) || j0 + kTile >= cnt)
+            for (int k = 0; k < 4; ++k) a[k].tile_end();
+          continue;
+        }
+      }
       const bool far = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) >= 9.9f * s_r2[sub];
       const bool tile_form = !split_only && __all(far);
       auto pair_tile = [&](int j) {
@@ -536,7 +723,8 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
 }
 
 void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, float* mass_range,
-                          int n_i, int n_j, int n_real, int soft, float eps, int split_only, hipStream_t s) {
+                          float* moments, int n_i, int n_j, int n_real, int soft, float eps, int split_only,
+                          hipStream_t s) {
   const dim3 grid(ceil_div(n_i, kGravBlock)), block(kGravBlock);
   if (soft == 0) {
     hipLaunchKernelGGL(k_gravity_mixed_spline, grid, block, 0, s, hi, lo, gvel, out, n_i, n_j);
@@ -550,8 +738,20 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
     const int chunk = ceil_div(ceil_div(n_j, kGravParts), kTile) * kTile;
     F4* part_hi = part;
     F4* part_lo = part + (size_t)kGravParts * n_i;
-    hipLaunchKernelGGL(k_gravity_mixed_plummer<kJTileMixed>, dim3(ceil_div(n_i, kGravBlock) * kGravParts), block, 0,
-                       s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range);
+    if ((split_only & 16) && moments) {
+      const int n_sub = ceil_div(n_j, kTile);
+      hipLaunchKernelGGL(k_subtile_moments, dim3(ceil_div(n_sub, kGravBlock / 64)), block, 0, s, hi, lo, moments,
+                         n_j, n_real, n_sub, split_only, mass_range);
+      CS_CHECK_LAUNCH();
+    }
+    if ((split_only & 16) && moments)
+      hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, true>), dim3(ceil_div(n_i, kGravBlock) * kGravParts),
+                         block, 0, s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range,
+                         moments);
+    else
+      hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, false>), dim3(ceil_div(n_i, kGravBlock) * kGravParts),
+                         block, 0, s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range,
+                         nullptr);
     CS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gravity_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part_hi, part_lo, out,
                        n_i, mass_range, split_only);

@@ -250,6 +250,84 @@ def test_mixed_gravity_against_fp64(cs, orc, equal):
     np.testing.assert_allclose(g[:, 3], go[:, 3], rtol=2e-6)
 
 
+@pytest.mark.parametrize("cfg,n,equal", [(3, 16384, True), (3, 16384, False), (5, 262144, True)])
+def test_mixed_gravity_far_field_against_fp64(cs, orc, cfg, n, equal):
+    """The optional multipole far field (grav_far = 1: order-4 moments of each
+    64-record sub-tile for wavefronts 16 sub-tile radii away) against FP64
+    gravity on the same positions, to the pairwise kernel's tolerance; equal
+    masses (unit-mass moments) and masses spread over a factor of 3.  The
+    uniform sphere needs N = 262144 for its sub-tiles to be small enough
+    (R ~ 0.08 in a sphere of radius 1); there the reference is the FP64 form
+    on the GPU (itself pinned to the FP64 oracle), the oracle's N^2 sum
+    being too slow on the CPU at that size."""
+    print(f"far field cfg={cfg} n={n} equal={equal}", flush=True)
+    e = cs.make_engine("mixed", cfg, n, 29, ph_sort=True)
+    v64 = e.get("vel64")
+    if not equal:
+        rng = np.random.default_rng(6)
+        v64[:n, 3] *= rng.uniform(0.5, 1.5, n)
+        e.set("vel64", v64)
+        vel = e.get("vel")
+        vel[:n, 3] = v64[:n, 3]
+        e.set("vel", vel)
+    e.run_pass(cs.GRAVITY)
+    g_direct = e.get("grav")[:n].astype(np.float64)
+    e.set_scalar("grav_far", 1)
+    e.run_pass(cs.GRAVITY)
+    g = e.get("grav")[:n].astype(np.float64)
+    assert not np.array_equal(g, g_direct), "the far field was not taken"
+    if n <= 16384:
+        o = orc.Oracle(64, e.params.copy())
+        P = e.get("pos").astype(np.float64)
+        P[:n, :3] = e.get("pos64")[:n, :3]
+        V = e.get("vel").astype(np.float64)
+        V[:n] = v64[:n]
+        o.set("pos", P)
+        o.set("vel", V)
+        o.run_pass(orc.GRAVITY)
+        go = o.get("grav")[:n]
+    else:
+        f = cs.Engine("fp64", e.params.copy())
+        P = f.get("pos")
+        P[:n, :3] = e.get("pos64")[:n, :3]
+        f.set("pos", P)
+        V = f.get("vel")
+        V[:n] = v64[:n]
+        f.set("vel", V)
+        f.run_pass(cs.GRAVITY)
+        go = f.get("grav")[:n]
+        f.close()
+    rms = np.sqrt((go[:, :3] ** 2).sum(1).mean())
+    err = np.linalg.norm(g[:, :3] - go[:, :3], axis=1) / np.maximum(np.linalg.norm(go[:, :3], axis=1), 1e-3 * rms)
+    assert err.max() < 1e-5, err.max()
+    np.testing.assert_allclose(g[:, 3], go[:, 3], rtol=2e-6)
+
+
+def test_mixed_far_field_trajectory_against_fp64(cs, orc):
+    """Three steps of the polytrope pair with the far field on: energy within
+    1e-6 of the FP64 oracle's per step, positions to 1e-6 of their extent."""
+    n = 16384
+    e = cs.make_engine("mixed", 3, n, 31, ph_sort=True)
+    e.set_scalar("grav_far", 1)
+    o = orc.Oracle(64, e.params.copy())
+    P = e.get("pos").astype(np.float64)
+    P[:n, :3] = e.get("pos64")[:n, :3]
+    H = e.get("hydro").astype(np.float64)
+    H[:n, 2] = e.get("u64")[:n]
+    o.set("pos", P)
+    o.set("vel", e.get("vel64"))
+    o.set("hydro", H)
+    e.init()
+    o.init()
+    e0 = abs(o.stats()["E"])
+    for _ in range(3):
+        e.step()
+        o.step()
+    assert abs(e.stats()["E"] - o.stats()["E"]) / e0 / 3 <= 1e-6
+    x, xo = e.get("pos64")[:n, :3], o.get("pos")[:n, :3]
+    assert np.abs(x - xo).max() <= 1e-6 * np.abs(xo).max()
+
+
 @pytest.mark.parametrize("form", ["plain", "mixed"])
 def test_gravity_stream_overlap_changes_nothing(cs, form):
     """Gravity on its own stream beside the SPH passes gives the same bits as
