@@ -36,11 +36,13 @@ WORKLOAD = "two n=1.5 polytropes (R=1, M=0.5 each), parabolic orbit q=0.5 from s
            "gas+radiation EOS mu=0.6 a=1e-3; 64+-8 neighbours; Plummer eps=0.002"
 
 
-def run_form(cs, form, n, seed, warmup, steps, timing=True, i_order=0):
+def run_form(cs, form, n, seed, warmup, steps, timing=True, i_order=0, scalars=None):
     # the initial conditions are put in Peano-Hilbert order (the reference's
     # periodic sort_bodies) so neighbour gathers read nearby memory
     e = cs.make_engine(form, 3, n, seed, ph_sort=True, i_order=i_order)
     e.set_scalar("timing", 1 if timing else 0)
+    for k, v in (scalars or {}).items():
+        e.set_scalar(k, v)
     e.init()
     energies = [e.stats()["E"]]
     for _ in range(warmup):
@@ -214,6 +216,7 @@ def scale_leg(cs, n, steps, seed):
     e = cs.make_engine("mixed", 5, n, seed, ph_sort=True, i_order=0)
     e.set_scalar("timing", 1)
     e.init()
+    e0 = e.stats()["E"]
     e.reset_timers()
     e.sync()
     t0 = time.perf_counter()
@@ -221,9 +224,24 @@ def scale_leg(cs, n, steps, seed):
     e.sync()
     wall = time.perf_counter() - t0
     timers = e.timers()
+    e_pair = e.stats()["E"]
     kbar = float(e.get("ngb_count")[:n].mean())
     walks = float(e.get("hash")[:, 0].astype(np.int64).sum())
     nd = e.nd
+    e.close()
+    # the same steps with the multipole far field (grav_far = 1)
+    e = cs.make_engine("mixed", 5, n, seed, ph_sort=True, i_order=0)
+    e.set_scalar("timing", 1)
+    e.set_scalar("grav_far", 1)
+    e.init()
+    e.reset_timers()
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.sync()
+    wall_far = time.perf_counter() - t0
+    t_far = e.timers()
+    e_far = e.stats()["E"]
     e.close()
     g_ms, g_calls = timers.get("gravity", (float("nan"), 1))
     g_avg = g_ms / max(g_calls, 1) / 1e3
@@ -233,7 +251,11 @@ def scale_leg(cs, n, steps, seed):
            "kernel_ms_per_step": {k: v[0] / steps for k, v in sorted(timers.items())},
            "sph_hbm": {"mean_list_length": kbar, "passes": sph_bandwidth(timers, steps, n, kbar, walks)},
            "gravity": {"kernel": "k_gravity_mixed_plummer", "launch_ms": g_avg * 1e3, "achieved": achieved,
-                       "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS}}
+                       "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS},
+           "far_field": {"value": n * steps / wall_far, "ms_per_step": wall_far / steps * 1e3,
+                         "gravity_ms": t_far["gravity"][0] / max(t_far["gravity"][1], 1),
+                         "energy_vs_pairwise": {"definition": "|E_far - E_pairwise| / |E(0)| / steps after the run",
+                                                "value": abs(e_far - e_pair) / abs(e0) / steps}}}
     print(json.dumps({"scale_progress": "mixed steps done", "ms_per_step": out["ms_per_step"]}), flush=True)
     p = cs.preset_params(5, n)
     pos, vel, hyd = cs.generate_ic(5, n, seed)
@@ -265,6 +287,8 @@ def main():
     ap.add_argument("--cpu-gravity-sample", type=int, default=4096)
     ap.add_argument("--i-order", type=int, default=0, help="0: memory order, 1: kd-tree heap order")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-far", action="store_true",
+                    help="skip the mixed form with the multipole far field (reported beside the headline)")
     ap.add_argument("--adjoint-n", type=int, default=65536)
     ap.add_argument("--adjoint-steps", type=int, default=20)
     ap.add_argument("--adjoint-ckpt", type=int, default=5)
@@ -301,6 +325,9 @@ def main():
     mixed = run_form(cs, "mixed", n, args.seed, W, K, i_order=args.i_order)
     plain = None if args.no_plain else run_form(cs, "plain", n, args.seed, W, K, i_order=args.i_order)
     fp64 = None if args.no_fp64 else run_form(cs, "fp64", n, args.seed, W, K, timing=False)
+    # the optional multipole far field (not pairwise, so not the headline)
+    far = None if args.no_far else run_form(cs, "mixed", n, args.seed, W, K, i_order=args.i_order,
+                                            scalars={"grav_far": 1})
 
     value = n * K / mixed["wall_s"]
     g_ms, g_calls = mixed["timers"].get("gravity", (float("nan"), 0))
@@ -355,6 +382,13 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profile("k_gravity"),
                      "per_launch": f"{FLOP_PER_PAIR} flop x {nd}^2 pairs",
                      "launch_ms": g_avg * 1e3},
+        "far_field": None if far is None else {
+            "what": "mixed form with the order-4 multipole far field of each 64-record sub-tile "
+                    "(grav_far = 1, taken beyond 8 sub-tile radii); the rest unchanged",
+            "value": n * K / far["wall_s"], "unit": "particle-updates/s", "ms_per_step": far["wall_s"] / K * 1e3,
+            "gravity_ms": far["timers"].get("gravity", (0, 1))[0] / max(far["timers"].get("gravity", (0, 1))[1], 1),
+            "over_pairwise_mixed": mixed["wall_s"] / far["wall_s"],
+            "energy_drift_vs_fp64": drift(far), "momentum_drift_vs_fp64": mom_drift(far)},
         "adjoint_over_forward": None,
     }
     if not args.no_adjoint:

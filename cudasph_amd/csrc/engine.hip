@@ -239,6 +239,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_unit_mass") return grav_unit_mass_;
   if (name == "grav_expand") return grav_expand_;
   if (name == "grav_far") return grav_far_;
+  if (name == "grav_far_ratio") return grav_far_ratio_;
   if (name == "overlap") return overlap_;
   if (name == "timing") return timers_.enabled;
   if (name == "cap") return (double)cap_;
@@ -269,6 +270,9 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     grav_expand_ = (int)v;
   } else if (name == "grav_far") {
     grav_far_ = (int)v;
+  } else if (name == "grav_far_ratio") {
+    if (!(v >= 4)) throw std::invalid_argument("grav_far_ratio must be at least 4");
+    grav_far_ratio_ = v;
   } else if (name == "overlap") {
     overlap_ = (int)v;
   } else if (name == "timing") {
@@ -537,7 +541,8 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
   TimerScope ts(timers_, "gravity", gs);
   if constexpr (kF32) {
     if (mixed()) {
-      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, gmass_, gmom_, nd_, nd_, n_, p_.grav_mode,
+      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, gmass_, gmom_, (float)grav_far_ratio_, nd_, nd_,
+                           n_, p_.grav_mode,
                            p_.grav_eps,
                            grav_split_only_ | (grav_unit_mass_ ? 0 : 2) | (grav_expand_ ? 0 : 4) |
                                (grav_far_ ? 16 : 0),

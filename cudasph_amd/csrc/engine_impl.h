@@ -193,6 +193,7 @@ class Engine final : public EngineBase {
   int grav_unit_mass_ = 1;    // mixed gravity: unit-mass pair body when every mass is equal
   int grav_expand_ = 1;       // mixed gravity: expanded r^2 in the equal-mass tile body
   int grav_far_ = 0;          // mixed gravity: order-4 multipole far field for distant sub-tiles
+  double grav_far_ratio_ = 8;  // ... taken beyond this many sub-tile radii
   int i_order_ = 1;   // 1: lanes take particles in kd-tree heap order, 0: in memory order
   long ngb_tot_ = 0;
   int overflow_reruns_ = 0;

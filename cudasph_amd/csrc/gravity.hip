@@ -228,9 +228,12 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
 // Q = tr M4, tr Q.  M^(1) = 0 about the centre of mass.  Truncation after
 // order 4 leaves a relative error ~ (R / |y|)^5 of the sub-tile's force
 // (R = max |e_j|): the far field is used for a (wavefront, sub-tile) only
-// when every lane has |y| >= kFarRatio R, where it is ~1e-7.
+// when every lane has |y| >= q R.  The default q = 8 (the engine's
+// grav_far_ratio) bounds that error near 1e-6 of one sub-tile's force, below
+// the FP32 pair arithmetic's: measured against FP64 at N = 2^20 the median
+// per-particle error rises by 5 % over the pairwise kernel's, the maximum
+// not at all; at q = 5 the maximum passes 1e-5.
 constexpr int kMomWords = 52;     // floats per sub-tile record, see k_subtile_moments
-constexpr float kFarRatio = 16.f;
 
 __device__ __forceinline__ float wave_sum(float v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -411,7 +414,8 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
                                                                        int n_j, int j_chunk, float eps2,
                                                                        int split_only,
                                                                        const float* __restrict__ mass_range,
-                                                                       const float* __restrict__ mom) {
+                                                                       const float* __restrict__ mom,
+                                                                       float far_q2) {
   // every mass equal (mass_range = {min, max} over the real particles): the
   // pair body drops its mass factor and k_gravity_combine scales by m once;
   // padding records sit at 1e30, where r^2 overflows and rsq returns 0, so
@@ -495,12 +499,12 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
       // (uniform per wavefront: no divergence); the squared test carries a
       // margin of 1.1 for the rounding of the squares
       if (FAR && far_on) {
-        // multipole far field when every lane is kFarRatio sub-tile radii
-        // from its centre of mass (uniform per wavefront)
+        // multipole far field when every lane is q sub-tile radii from its
+        // centre of mass (far_q2 = q^2; uniform per wavefront)
         const float* M = &s_mom[sub][0];
         const float yx = dx - M[0], yy = dy - M[1], yz = dz - M[2];
         const float y2 = __builtin_fmaf(yx, yx, __builtin_fmaf(yy, yy, yz * yz));
-        if (__all(y2 >= kFarRatio * kFarRatio * M[3])) {
+        if (__all(y2 >= far_q2 * M[3])) {
           float gx, gy, gz, pt;
           far_field(M, yx, yy, yz, eps2, gx, gy, gz, pt);
           a[0].part += gx;
@@ -723,8 +727,8 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
 }
 
 void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, float* mass_range,
-                          float* moments, int n_i, int n_j, int n_real, int soft, float eps, int split_only,
-                          hipStream_t s) {
+                          float* moments, float far_ratio, int n_i, int n_j, int n_real, int soft, float eps,
+                          int split_only, hipStream_t s) {
   const dim3 grid(ceil_div(n_i, kGravBlock)), block(kGravBlock);
   if (soft == 0) {
     hipLaunchKernelGGL(k_gravity_mixed_spline, grid, block, 0, s, hi, lo, gvel, out, n_i, n_j);
@@ -747,11 +751,11 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
     if ((split_only & 16) && moments)
       hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, true>), dim3(ceil_div(n_i, kGravBlock) * kGravParts),
                          block, 0, s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range,
-                         moments);
+                         moments, far_ratio * far_ratio);
     else
       hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, false>), dim3(ceil_div(n_i, kGravBlock) * kGravParts),
                          block, 0, s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range,
-                         nullptr);
+                         nullptr, 0.f);
     CS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gravity_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part_hi, part_lo, out,
                        n_i, mass_range, split_only);

@@ -52,10 +52,11 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
 // mass_range: scratch of 2 + 2 * ceil(n_real / 256) floats ({min, max} of the
 // real particles' masses, then per-block partials)
 // moments: scratch of 52 floats per 64-record j sub-tile (the optional
-// multipole far field, split_only bit 4; may be null otherwise)
+// multipole far field, split_only bit 4, taken where every lane of a
+// wavefront is far_ratio sub-tile radii away; may be null otherwise)
 void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, float* mass_range,
-                          float* moments, int n_i, int n_j, int n_real, int soft, float eps, int split_only,
-                          hipStream_t s);
+                          float* moments, float far_ratio, int n_i, int n_j, int n_real, int soft, float eps,
+                          int split_only, hipStream_t s);
 
 // ---- sort.hip ----
 // exclusive scan of in[0..n) into out[0..n]; out[n] = total.  mode 1 scans

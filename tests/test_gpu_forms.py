@@ -253,7 +253,7 @@ def test_mixed_gravity_against_fp64(cs, orc, equal):
 @pytest.mark.parametrize("cfg,n,equal", [(3, 16384, True), (3, 16384, False), (5, 262144, True)])
 def test_mixed_gravity_far_field_against_fp64(cs, orc, cfg, n, equal):
     """The optional multipole far field (grav_far = 1: order-4 moments of each
-    64-record sub-tile for wavefronts 16 sub-tile radii away) against FP64
+    64-record sub-tile for wavefronts 8 sub-tile radii away) against FP64
     gravity on the same positions, to the pairwise kernel's tolerance; equal
     masses (unit-mass moments) and masses spread over a factor of 3.  The
     uniform sphere needs N = 262144 for its sub-tiles to be small enough
