@@ -122,7 +122,9 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
     gvel32_.alloc(nd_);
     gpart_.alloc(2 * 8 * (size_t)nd_);
     gmass_.alloc(2 + 2 * (size_t)ceil_div(n_, kBlock));
-    gmom_.alloc(52 * (size_t)ceil_div(nd_, 64));
+    // far-field records: 64-record sub-tiles, then cells of 16384, 4096, 1024, 256
+    gmom_.alloc(52 * ((size_t)ceil_div(nd_, 64) + ceil_div(nd_, 16384) + ceil_div(nd_, 4096) + ceil_div(nd_, 1024) +
+                      ceil_div(nd_, 256)));
     CS_HIP(hipMemsetAsync(pos_lo_, 0, pos_lo_.bytes(), s_));
     CS_HIP(hipMemsetAsync(pos64_, 0, pos64_.bytes(), s_));
     CS_HIP(hipMemsetAsync(vel64_, 0, vel64_.bytes(), s_));

@@ -240,8 +240,9 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
-// record of sub-tile s: [0..2] centre-of-mass offset from the sub-tile's
-// middle record (the tile form's centre), [3] R^2, [4] M0, [5] tr M2,
+// record of sub-tile s: [0..2] centre-of-mass offset from the origin record
+// (the middle real record; its coordinates in [48..50]), [3] R^2, [4] M0
+// (0: no real record, the sub-tile is skipped), [5] tr M2,
 // [6] tr Q, [8..13] M2 (xx xy xz yy yz zz), [14..16] t3, [17..22] Q,
 // [23..32] M3 (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz), [33..47] M4
 // (xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz xyzz xzzz yyyy yyyz yyzz yzzz
@@ -256,7 +257,9 @@ __global__ __launch_bounds__(kGravBlock) void k_subtile_moments(const F4* __rest
   const int lane = threadIdx.x & 63;
   if (s >= n_sub) return;   // whole wavefronts
   const int j0 = s * kTile, nj = min(kTile, n_j - j0);
-  const F4 c = hi[j0 + nj / 2];
+  // origin: the middle real record (a padding record sits at 1e30)
+  const int nr = max(0, min(nj, n_real - j0));
+  const F4 c = hi[j0 + nr / 2];
   const int j = j0 + lane;
   const bool valid = lane < nj && j < n_real;
   float dx = 0.f, dy = 0.f, dz = 0.f, m = 0.f;
@@ -311,7 +314,111 @@ __global__ __launch_bounds__(kGravBlock) void k_subtile_moments(const F4* __rest
     for (int k = 0; k < 10; ++k) r[23 + k] = M3[k];
 #pragma unroll
     for (int k = 0; k < 15; ++k) r[33 + k] = M4[k];
-    r[48] = r[49] = r[50] = r[51] = 0.f;
+    r[48] = c.x;
+    r[49] = c.y;
+    r[50] = c.z;
+    r[51] = 0.f;
+  }
+}
+
+// Cells of C = 256, 1024, 4096 and 16384 records (the hierarchy above the
+// sub-tiles): one block per cell, the same record layout.  Sums in FP64, the
+// whole cell at once: a cell holds up to 16384 records, whose FP32 sums
+// would round at ~1e-5.
+constexpr int kFarLevels = 4;
+constexpr int kCellSize[kFarLevels] = {16384, 4096, 1024, 256};   // coarse to fine
+
+__device__ __forceinline__ double block_sum(double v, double* s_red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < kGravBlock / 64; ++w) t += s_red[w];   // fixed order, every thread the same
+  return t;
+}
+
+__global__ __launch_bounds__(kGravBlock) void k_cell_moments(const F4* __restrict__ hi, const F4* __restrict__ lo,
+                                                             float* __restrict__ mom, int n_j, int n_real, int C,
+                                                             int flags, const float* __restrict__ mass_range) {
+  __shared__ double s_red[kGravBlock / 64];
+  const bool uniform = (flags & 2) == 0 && mass_range[0] == mass_range[1];
+  const int cell = blockIdx.x;
+  const int j0 = cell * C, nj = min(C, n_j - j0);
+  // origin: the middle real record (a padding record sits at 1e30)
+  const int nr = max(0, min(nj, n_real - j0));
+  const F4 c = hi[j0 + nr / 2];
+  // pass 1: mass and centre of mass
+  double m0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int q = threadIdx.x; q < nj; q += kGravBlock) {
+    const int j = j0 + q;
+    if (j >= n_real) continue;
+    const F4 h = hi[j], l = lo[j];
+    const double m = uniform ? 1.0 : (double)h.w;
+    m0 += m;
+    sx += m * (double)((h.x - c.x) + l.x);
+    sy += m * (double)((h.y - c.y) + l.y);
+    sz += m * (double)((h.z - c.z) + l.z);
+  }
+  m0 = block_sum(m0, s_red);
+  sx = block_sum(sx, s_red);
+  sy = block_sum(sy, s_red);
+  sz = block_sum(sz, s_red);
+  const double mx = m0 > 0.0 ? sx / m0 : 0.0, my = m0 > 0.0 ? sy / m0 : 0.0, mz = m0 > 0.0 ? sz / m0 : 0.0;
+  // pass 2: radius and central moments
+  double acc[31];
+  for (int k = 0; k < 31; ++k) acc[k] = 0.0;
+  float r2 = 0.f;
+  for (int q = threadIdx.x; q < nj; q += kGravBlock) {
+    const int j = j0 + q;
+    if (j >= n_real) continue;
+    const F4 h = hi[j], l = lo[j];
+    const double m = uniform ? 1.0 : (double)h.w;
+    const double ex = (double)((h.x - c.x) + l.x) - mx, ey = (double)((h.y - c.y) + l.y) - my,
+                 ez = (double)((h.z - c.z) + l.z) - mz;
+    r2 = fmaxf(r2, (float)(ex * ex + ey * ey + ez * ez));
+    const double xx = m * ex * ex, xy = m * ex * ey, xz = m * ex * ez, yy = m * ey * ey, yz = m * ey * ez,
+                 zz = m * ez * ez;
+    const double xxx = xx * ex, xxy = xx * ey, xxz = xx * ez, xyy = xy * ey, xyz = xy * ez, xzz = xz * ez,
+                 yyy = yy * ey, yyz = yy * ez, yzz = yz * ez, zzz = zz * ez;
+    const double t[31] = {xx, xy, xz, yy, yz, zz, xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz,
+                          xxx * ex, xxx * ey, xxx * ez, xxy * ey, xxy * ez, xxz * ez, xyy * ey, xyy * ez,
+                          xyz * ez, xzz * ez, yyy * ey, yyy * ez, yyz * ez, yzz * ez, zzz * ez};
+    for (int k = 0; k < 31; ++k) acc[k] += t[k];
+  }
+  for (int k = 0; k < 31; ++k) acc[k] = block_sum(acc[k], s_red);
+  for (int o = 32; o > 0; o >>= 1) r2 = fmaxf(r2, __shfl_xor(r2, o));
+  __syncthreads();
+  float* s_r2 = reinterpret_cast<float*>(s_red);
+  if ((threadIdx.x & 63) == 0) s_r2[threadIdx.x >> 6] = r2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kGravBlock / 64; ++w) r2 = fmaxf(r2, s_r2[w]);
+    const double* M2 = acc;
+    const double* M3 = acc + 6;
+    const double* M4 = acc + 16;
+    const double Q[6] = {M4[0] + M4[3] + M4[5], M4[1] + M4[6] + M4[8], M4[2] + M4[7] + M4[9],
+                         M4[3] + M4[10] + M4[12], M4[4] + M4[11] + M4[13], M4[5] + M4[12] + M4[14]};
+    float* r = mom + (size_t)cell * kMomWords;
+    r[0] = (float)mx;
+    r[1] = (float)my;
+    r[2] = (float)mz;
+    r[3] = r2;
+    r[4] = (float)m0;
+    r[5] = (float)(M2[0] + M2[3] + M2[5]);
+    r[6] = (float)(Q[0] + Q[3] + Q[5]);
+    r[7] = 0.f;
+    for (int k = 0; k < 6; ++k) r[8 + k] = (float)M2[k];
+    r[14] = (float)(M3[0] + M3[3] + M3[5]);
+    r[15] = (float)(M3[1] + M3[6] + M3[8]);
+    r[16] = (float)(M3[2] + M3[7] + M3[9]);
+    for (int k = 0; k < 6; ++k) r[17 + k] = (float)Q[k];
+    for (int k = 0; k < 10; ++k) r[23 + k] = (float)M3[k];
+    for (int k = 0; k < 15; ++k) r[33 + k] = (float)M4[k];
+    r[48] = c.x;
+    r[49] = c.y;
+    r[50] = c.z;
+    r[51] = 0.f;
   }
 }
 
@@ -415,7 +522,9 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
                                                                        int split_only,
                                                                        const float* __restrict__ mass_range,
                                                                        const float* __restrict__ mom,
-                                                                       float far_q2) {
+                                                                       float far_q2,
+                                                                       const float* __restrict__ cells,
+                                                                       int n_real) {
   // every mass equal (mass_range = {min, max} over the real particles): the
   // pair body drops its mass factor and k_gravity_combine scales by m once;
   // padding records sit at 1e30, where r^2 overflows and rsq returns 0, so
@@ -437,11 +546,63 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
   const int ii = i < n_i ? i : n_i - 1;
   const int lane = threadIdx.x & 63;
   const F4 pi = hi[ii], li = lo[ii];
+  // padding lanes (records at 1e30) take no far-field term: its products
+  // would overflow there; they pass every acceptance test
+  const bool real_i = !FAR || i < n_real;
   GAcc<float, 1> a[4];
+  auto add_far = [&](const float* M, float yx, float yy, float yz) {
+    float gx, gy, gz, pt;
+    far_field(M, yx, yy, yz, eps2, gx, gy, gz, pt);
+    if (real_i) {
+      a[0].part += gx;
+      a[1].part += gy;
+      a[2].part += gz;
+      a[3].part += pt;
+    }
+  };
   const int j_begin = min(part * j_chunk, n_j), j_end = min(j_begin + j_chunk, n_j);
+  int skip_end = j_begin;   // this wavefront is done with the records below
   for (int t0 = j_begin; t0 < j_end; t0 += JT) {
     const int cnt = min(JT, j_end - t0);
-    __syncthreads();
+    bool need = true;
+    if constexpr (FAR) {
+      // the coarsest cell starting here, inside this part (or the array's
+      // last, partial cell), that every lane sees from beyond q cell radii:
+      // one expansion for all of it
+      if (far_on) {
+        need = t0 >= skip_end;
+        const float* Mc = nullptr;
+        int Cc = 0;
+        size_t off = 0;
+        for (int L = 0; L < kFarLevels; ++L) {
+          const int C = kCellSize[L];
+          if (need && !Mc && t0 % C == 0 && (t0 + C <= j_end || j_end == n_j)) {
+            const float* M = cells + off + (size_t)(t0 / C) * kMomWords;
+            const float yx = ((pi.x - M[48]) + li.x) - M[0], yy = ((pi.y - M[49]) + li.y) - M[1],
+                        yz = ((pi.z - M[50]) + li.z) - M[2];
+            const float y2 = __builtin_fmaf(yx, yx, __builtin_fmaf(yy, yy, yz * yz));
+            if (M[4] == 0.f || __all(!real_i || y2 >= far_q2 * M[3])) {
+              Mc = M;
+              Cc = C;
+            }
+          }
+          off += (size_t)((n_j + C - 1) / C) * kMomWords;
+        }
+        if (Mc) {   // one expansion, whichever level accepted (none for an empty cell)
+          if (Mc[4] != 0.f)
+            add_far(Mc, ((pi.x - Mc[48]) + li.x) - Mc[0], ((pi.y - Mc[49]) + li.y) - Mc[1],
+                    ((pi.z - Mc[50]) + li.z) - Mc[2]);
+          for (int k = 0; k < 4; ++k) a[k].tile_end();
+          skip_end = min(t0 + Cc, j_end);
+          need = false;
+        }
+      }
+      // the barrier that orders the previous tile's LDS reads before this
+      // tile's staging also tells whether any wavefront needs the records
+      if (!__syncthreads_or(need)) continue;
+    } else {
+      __syncthreads();
+    }
     // stage: this wavefront's sub-tiles, each relative to its middle
     // record, and each sub-tile's squared radius about that record
     for (int q = threadIdx.x; q < JT; q += kGravBlock) {
@@ -472,6 +633,7 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
       }
     }
     __syncthreads();
+    if (FAR && !need) continue;   // a cell covered this tile for this wavefront
     auto pair = [&](float rx, float ry, float rz, float mj) {
       const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
       const float inv = __builtin_amdgcn_rsqf(d2);
@@ -502,15 +664,11 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* 
         // multipole far field when every lane is q sub-tile radii from its
         // centre of mass (far_q2 = q^2; uniform per wavefront)
         const float* M = &s_mom[sub][0];
-        const float yx = dx - M[0], yy = dy - M[1], yz = dz - M[2];
+        const float yx = ((pi.x - M[48]) + li.x) - M[0], yy = ((pi.y - M[49]) + li.y) - M[1],
+                    yz = ((pi.z - M[50]) + li.z) - M[2];
         const float y2 = __builtin_fmaf(yx, yx, __builtin_fmaf(yy, yy, yz * yz));
-        if (__all(y2 >= far_q2 * M[3])) {
-          float gx, gy, gz, pt;
-          far_field(M, yx, yy, yz, eps2, gx, gy, gz, pt);
-          a[0].part += gx;
-          a[1].part += gy;
-          a[2].part += gz;
-          a[3].part += pt;
+        if (M[4] == 0.f || __all(!real_i || y2 >= far_q2 * M[3])) {
+          if (M[4] != 0.f) add_far(M, yx, yy, yz);
           if ((sub & 1) || j0 + kTile >= cnt)
             for (int k = 0; k < 4; ++k) a[k].tile_end();
           continue;
@@ -742,20 +900,31 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
     const int chunk = ceil_div(ceil_div(n_j, kGravParts), kTile) * kTile;
     F4* part_hi = part;
     F4* part_lo = part + (size_t)kGravParts * n_i;
+    float* cells = nullptr;
     if ((split_only & 16) && moments) {
       const int n_sub = ceil_div(n_j, kTile);
       hipLaunchKernelGGL(k_subtile_moments, dim3(ceil_div(n_sub, kGravBlock / 64)), block, 0, s, hi, lo, moments,
                          n_j, n_real, n_sub, split_only, mass_range);
       CS_CHECK_LAUNCH();
+      // the cell levels follow the sub-tile records in the same buffer
+      cells = moments + (size_t)n_sub * kMomWords;
+      size_t off = 0;
+      for (int L = 0; L < kFarLevels; ++L) {
+        const int n_cells = ceil_div(n_j, kCellSize[L]);
+        hipLaunchKernelGGL(k_cell_moments, dim3(n_cells), block, 0, s, hi, lo, cells + off, n_j, n_real,
+                           kCellSize[L], split_only, mass_range);
+        CS_CHECK_LAUNCH();
+        off += (size_t)n_cells * kMomWords;
+      }
     }
     if ((split_only & 16) && moments)
       hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, true>), dim3(ceil_div(n_i, kGravBlock) * kGravParts),
                          block, 0, s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range,
-                         moments, far_ratio * far_ratio);
+                         moments, far_ratio * far_ratio, cells, n_real);
     else
       hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, false>), dim3(ceil_div(n_i, kGravBlock) * kGravParts),
                          block, 0, s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range,
-                         nullptr, 0.f);
+                         nullptr, 0.f, nullptr, n_real);
     CS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gravity_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part_hi, part_lo, out,
                        n_i, mass_range, split_only);

@@ -51,9 +51,10 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
 // part: scratch of 2 * 8 * n_i records (per-j-part partial sums, Plummer only);
 // mass_range: scratch of 2 + 2 * ceil(n_real / 256) floats ({min, max} of the
 // real particles' masses, then per-block partials)
-// moments: scratch of 52 floats per 64-record j sub-tile (the optional
-// multipole far field, split_only bit 4, taken where every lane of a
-// wavefront is far_ratio sub-tile radii away; may be null otherwise)
+// moments: scratch of 52 floats per 64-record j sub-tile followed by 52 per
+// cell of 16384, 4096, 1024 and 256 records (the optional multipole far
+// field, split_only bit 4, taken where every lane of a wavefront is
+// far_ratio cell radii away; may be null otherwise)
 void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, float* mass_range,
                           float* moments, float far_ratio, int n_i, int n_j, int n_real, int soft, float eps,
                           int split_only, hipStream_t s);

@@ -250,7 +250,8 @@ def test_mixed_gravity_against_fp64(cs, orc, equal):
     np.testing.assert_allclose(g[:, 3], go[:, 3], rtol=2e-6)
 
 
-@pytest.mark.parametrize("cfg,n,equal", [(3, 16384, True), (3, 16384, False), (5, 262144, True)])
+@pytest.mark.parametrize("cfg,n,equal", [(3, 16384, True), (3, 16384, False), (3, 16000, True),
+                                         (5, 262144, True)])
 def test_mixed_gravity_far_field_against_fp64(cs, orc, cfg, n, equal):
     """The optional multipole far field (grav_far = 1: order-4 moments of each
     64-record sub-tile for wavefronts 8 sub-tile radii away) against FP64
@@ -259,7 +260,8 @@ def test_mixed_gravity_far_field_against_fp64(cs, orc, cfg, n, equal):
     uniform sphere needs N = 262144 for its sub-tiles to be small enough
     (R ~ 0.08 in a sphere of radius 1); there the reference is the FP64 form
     on the GPU (itself pinned to the FP64 oracle), the oracle's N^2 sum
-    being too slow on the CPU at that size."""
+    being too slow on the CPU at that size.  N = 16000 has padding records
+    (and padding lanes), which must take no far-field term."""
     print(f"far field cfg={cfg} n={n} equal={equal}", flush=True)
     e = cs.make_engine("mixed", cfg, n, 29, ph_sort=True)
     v64 = e.get("vel64")
@@ -276,6 +278,7 @@ def test_mixed_gravity_far_field_against_fp64(cs, orc, cfg, n, equal):
     e.run_pass(cs.GRAVITY)
     g = e.get("grav")[:n].astype(np.float64)
     assert not np.array_equal(g, g_direct), "the far field was not taken"
+    assert np.isfinite(e.get("grav")).all(), "padding records must stay finite"
     if n <= 16384:
         o = orc.Oracle(64, e.params.copy())
         P = e.get("pos").astype(np.float64)
