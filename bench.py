@@ -383,8 +383,9 @@ def main():
                      "per_launch": f"{FLOP_PER_PAIR} flop x {nd}^2 pairs",
                      "launch_ms": g_avg * 1e3},
         "far_field": None if far is None else {
-            "what": "mixed form with the order-4 multipole far field of each 64-record sub-tile "
-                    "(grav_far = 1, taken beyond 8 sub-tile radii); the rest unchanged",
+            "what": "mixed form with the order-4 multipole far field (grav_far = 1): per wavefront, "
+                    "the coarsest cell of 16384/4096/1024/256 records or 64-record sub-tile seen from "
+                    "beyond 8 of its radii takes one expansion; the rest unchanged",
             "value": n * K / far["wall_s"], "unit": "particle-updates/s", "ms_per_step": far["wall_s"] / K * 1e3,
             "gravity_ms": far["timers"].get("gravity", (0, 1))[0] / max(far["timers"].get("gravity", (0, 1))[1], 1),
             "over_pairwise_mixed": mixed["wall_s"] / far["wall_s"],

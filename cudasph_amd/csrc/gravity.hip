@@ -513,8 +513,14 @@ __device__ __forceinline__ void far_field(const float* __restrict__ M, float yx,
 // streams the whole j-set and re-fetches it from HBM.  Each block writes its
 // compensated (hi, lo) partial sums; k_gravity_combine adds the parts in
 // order.
+// The far-field instantiation holds the 48 moment words of an expansion in
+// VGPRs next to the pairwise state: left alone the compiler takes 127
+// registers (4 wavefronts per SIMD); capped at 5 wavefronts it fits 93
+// registers without spilling.  The pairwise instantiation is left unbounded
+// (63 registers, 8 wavefronts).
 template <int JT, bool FAR>
-__global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_plummer(const F4* __restrict__ hi,
+__global__ __launch_bounds__(kGravBlock) __attribute__((amdgpu_waves_per_eu(FAR ? 5 : 1)))
+void k_gravity_mixed_plummer(const F4* __restrict__ hi,
                                                                        const F4* __restrict__ lo,
                                                                        F4* __restrict__ part_hi,
                                                                        F4* __restrict__ part_lo, int n_i,

@@ -178,8 +178,9 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
  * grav_unit_mass (0: the general-mass body even when every mass is equal),
  * grav_expand (0: the equal-mass tile body on r instead of the expanded one).
  * Mixed-form gravity options: grav_far (1: an order-4 multipole far field for
- * each 64-record sub-tile that a whole wavefront sees from beyond
- * grav_far_ratio sub-tile radii, default 8; not pairwise, so off by default).
+ * each cell of 16384, 4096, 1024 or 256 records, or 64-record sub-tile, that a
+ * whole wavefront sees from beyond grav_far_ratio of its radii, default 8,
+ * coarsest first; not pairwise, so off by default).
  * Read only: ngb_tot, overflow_reruns, cap, real_bytes, form.  Unknown names
  * return NaN (get) or -1 with sph_last_error() (set). */
 double sph_engine_get_scalar(void* e, const char* name);
