@@ -5,8 +5,9 @@ mixed-precision form vs the plain FP32 drop-in, with energy drift measured
 against an FP64 run of the same steps.
 
 `value` is the mixed form's whole-step throughput, N * K / wall(K steps), with
-the state resident in HBM (the per-step host kd-tree build and its transfers
-are inside the timed region, as in the reference's step()).
+the state resident in HBM (the per-step kd-tree build is inside the timed
+region, as in the reference's step(): on the device in the mixed form, on the
+host with its transfers in the plain form).
 
 Extra objects on the JSON line:
   roofline      gravity kernel (the dominant kernel) of the mixed form:

@@ -85,6 +85,7 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
   CS_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
   CS_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
   cap_ = 100L * nd_;  // ngb_tot_max (read_state.cpp:29)
+  tree_gpu_ = form == SPH_FORM_MIXED;   // the plain and FP64 forms keep the reference-exact host build
 
   for (auto* b : {&pos_, &vel_, &hydro_, &dots_, &bdots_, &hydro2_, &grav_, &gpos_, &gvel_, &old_vu_, &tmp4_})
     b->alloc(nd_);
@@ -243,6 +244,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_far") return grav_far_;
   if (name == "grav_far_ratio") return grav_far_ratio_;
   if (name == "overlap") return overlap_;
+  if (name == "tree_gpu") return tree_gpu_;
   if (name == "timing") return timers_.enabled;
   if (name == "cap") return (double)cap_;
   if (name == "real_bytes") return (double)sizeof(T);
@@ -277,6 +279,8 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     grav_far_ratio_ = v;
   } else if (name == "overlap") {
     overlap_ = (int)v;
+  } else if (name == "tree_gpu") {
+    tree_gpu_ = (int)v;
   } else if (name == "timing") {
     sync();
     timers_.enabled = v != 0;
@@ -302,13 +306,23 @@ void Engine<T>::stats(double* o) {
 // ---------------------------------------------------------------------------
 // passes
 
-// kd-tree build on the host (sph_accelerations.cpp:14-33), reference-exact.
-// tree_fetch enqueues the position download; tree_build waits for it, builds
-// the tree and enqueues the upload, so GPU work queued in between (gravity
-// under Plummer softening) runs while the host builds.
+// kd-tree build (sph_accelerations.cpp:14-33).
+// Host build (tree_gpu 0), reference-exact: tree_fetch enqueues the position
+// download; tree_build waits for it, builds the tree and enqueues the upload,
+// so GPU work queued in between (gravity under Plummer softening) runs while
+// the host builds.  Device build (tree_gpu 1): tree_fetch enqueues it, ahead
+// of that GPU work, and tree_build only collects dt.
 template <class T>
 void Engine<T>::tree_fetch() {
-  CS_HIP(hipMemcpyAsync(h_pos_.p, pos_, (size_t)nd_ * sizeof(V4<T>), hipMemcpyDeviceToHost, s_));
+  if (tree_gpu_) {
+    const long need = tree_work_ints(n_);
+    if ((long)tree_work_.n < need) tree_work_.alloc(need);
+    TimerScope ts(timers_, "tree_build", s_);
+    launch_tree_build<T>(pos_, n_, nd_, tree_, hash_, i_order_, tree_work_, pairs_, pairs_alt_, sort_tmp_, scan_tmp_,
+                         s_);
+  } else {
+    CS_HIP(hipMemcpyAsync(h_pos_.p, pos_, (size_t)nd_ * sizeof(V4<T>), hipMemcpyDeviceToHost, s_));
+  }
   CS_HIP(hipMemcpyAsync(&h_small_[0], dt_cur_, 8, hipMemcpyDeviceToHost, s_));
   CS_HIP(hipEventRecord(fetched_, s_));
 }
@@ -317,6 +331,7 @@ template <class T>
 void Engine<T>::tree_build() {
   CS_HIP(hipEventSynchronize(fetched_));
   std::memcpy(&dt_, &h_small_[0], 8);
+  if (tree_gpu_) return;
   const auto t0 = std::chrono::steady_clock::now();
   build_kd_tree<T>(h_pos_.p, n_, nd_, p_.n_threads, h_tree_.p, h_hash_.p);
   if (i_order_ == 0)   // lanes take particles in memory order (the reference's identity hash)

@@ -195,6 +195,11 @@ class Engine final : public EngineBase {
   int grav_far_ = 0;          // mixed gravity: order-4 multipole far field for distant sub-tiles
   double grav_far_ratio_ = 8;  // ... taken beyond this many sub-tile radii
   int i_order_ = 1;   // 1: lanes take particles in kd-tree heap order, 0: in memory order
+  // 1: the kd-tree is built on the device (same tree without coordinate
+  // ties; with ties the same neighbour sets; the mixed form's default);
+  // 0: on the host, bit-identical to the reference's quicksort build
+  int tree_gpu_ = 0;
+  DBuf<int> tree_work_;   // allocated on first use of the device build
   long ngb_tot_ = 0;
   int overflow_reruns_ = 0;
 };

@@ -91,6 +91,17 @@ template <class T>
 void launch_ph_keys(const V4<T>* pos, int n, int bits, uint32_t* key_hi, uint32_t* key_lo, int* scratch,
                     hipStream_t s);
 
+// ---- tree.hip ----
+// The kd-tree of the first n positions (kd_tree.cpp:39-128) built on the
+// device: tree[0..n_dev+1] and hash[0..n_dev) as the host builder writes them
+// (heap_order 1: hash in the nodes' heap order, 0: identity).  work holds
+// tree_work_ints(n) ints; kv and kv_alt n records each; sort_scratch
+// radix_scratch_ints(n) and scan_scratch scan_scratch_ints(n) ints.
+long tree_work_ints(long n);
+template <class T>
+void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash, int heap_order, int* work,
+                       KV* kv, KV* kv_alt, int* sort_scratch, int* scan_scratch, hipStream_t s);
+
 // ---- integrate.hip ----
 template <class T>
 void launch_gravity_pack(const V4<T>* pos, const V4<T>* vel, V4<T>* gpos, V4<T>* gvel, int n, int n_dev,

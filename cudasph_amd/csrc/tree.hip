@@ -1,0 +1,261 @@
+// Device build of the reference's kd-tree (kd_tree.cpp:39-128, packing
+// sph_accelerations.cpp:21-33).
+//
+// The tree is the left-balanced implicit tree: node p holds one particle, its
+// children are 2p and 2p+1, the split axis of depth d is d % 3, and the node's
+// particle is the one at rank size(2p) (the size of its left subtree) of its
+// subtree's particles along that axis.  Every node of the nodes 1..n exists,
+// so the tree's shape, every subtree's size and where it starts among the
+// nodes of a level are closed forms of n (subtree_size, subtree_start).
+//
+// The build is level by level, on three lists of particle ids, one per axis,
+// each sorted once by that coordinate (stable LSD radix sort on an
+// order-preserving integer image of the coordinate) and kept grouped by
+// subtree: at level L every particle not yet placed carries the heap index t
+// of the level-L subtree that holds it, and each list holds the unplaced
+// particles subtree by subtree (t ascending), each subtree's run still in
+// its axis's order.  One level:
+//  * the list of the split axis L % 3 holds each subtree's particles in split
+//    order, so the particle at rank size(2t) of the run becomes node t, the
+//    ones before go to subtree 2t and the ones after to 2t+1 -- both runs stay
+//    contiguous and in order, shifted by the closed-form count of nodes placed
+//    before them;
+//  * the other two lists are partitioned stably: a particle going to 2t is
+//    placed after the earlier ones of its run going to 2t (an exclusive scan
+//    of "goes left" flags), one going to 2t+1 after the earlier ones going
+//    right (its rank in the run minus those going left and the one placed).
+// Levels: floor(log2 n) + 1.  Work per level is O(n); nothing is sorted after
+// the first three sorts.
+//
+// Ties in a coordinate: the reference's first-element-pivot quicksort decides
+// which of two equal-coordinate particles becomes the node; here the order of
+// the first sort (ascending particle id) does.  Either way every particle of a
+// left subtree has coordinate <= the node's and every one of a right subtree
+// >= it, which is all the walks' pruning (dev_solve_range.cu:48-61) uses, so
+// the neighbour sets are the same; without ties the two trees are identical.
+#include "common.h"
+#include "kernels.h"
+
+namespace cudasph {
+namespace {
+
+// nodes in the subtree of heap node t of the complete tree of n nodes
+__device__ __forceinline__ int subtree_size(int t, int n) {
+  int size = 0;
+  long long lo = t, hi = t;
+  while (lo <= n) {
+    size += (int)((hi < n ? hi : n) - lo + 1);
+    lo <<= 1;
+    hi = (hi << 1) | 1;
+  }
+  return size;
+}
+
+// among the nodes of levels >= L, laid out subtree by subtree, the offset of
+// the subtree of level-L node t: the nodes below the level-L nodes left of t
+__device__ __forceinline__ int subtree_start(int t, int L, int n) {
+  int s = 0;
+  long long a = 1ll << L, b = t;
+  while (a <= n) {
+    s += (int)((b < (long long)n + 1 ? b : (long long)n + 1) - a);
+    a <<= 1;
+    b <<= 1;
+  }
+  return s;
+}
+
+// order-preserving unsigned images of a coordinate (IEEE: flip all bits of a
+// negative value, the sign bit of a positive one)
+__device__ __forceinline__ uint32_t ordered_bits(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ uint64_t ordered_bits(double x) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+template <class T>
+__device__ __forceinline__ T coord(const V4<T>& p, int axis) {
+  return axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
+}
+
+__device__ __forceinline__ float node_word(float, int bits) { return __int_as_float(bits); }
+__device__ __forceinline__ double node_word(double, int bits) { return (double)bits; }
+
+// key word `word` (0: low 32 bits, 1: high 32 bits of a double's image) of
+// the coordinate of particle `value` (the value of the record, so a second
+// pass re-keys records already ordered by the first)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_tree_keys(const V4<T>* __restrict__ pos, KV* __restrict__ kv, int n,
+                                                       int axis, int word, int first) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int id = first ? i : (int)kv[i].value;
+  const auto b = ordered_bits(coord(pos[id], axis));
+  kv[i] = {(uint32_t)(word ? (uint64_t)b >> 32 : (uint64_t)b & 0xffffffffu), (uint32_t)id};
+}
+
+__global__ __launch_bounds__(kBlock) void k_tree_ids(const KV* __restrict__ kv, int* __restrict__ list,
+                                                      int* __restrict__ tag, int n, int set_tag) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  list[i] = (int)kv[i].value;
+  if (set_tag) tag[i] = 1;
+}
+
+// fixed slots (sph_accelerations.cpp:21-33, read_state.cpp:85-88): slot 0 and
+// slot n_dev+1 zero, slots n+1..n_dev far sentinels with split 3; the
+// identity hash, or the padding part of the heap-order hash
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_tree_fixed(V4<T>* __restrict__ tree, KV* __restrict__ hash, int n,
+                                                        int n_dev, int heap_order) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n_dev + 2) return;
+  if (t == 0 || t == n_dev + 1) tree[t] = {T(0), T(0), T(0), T(0)};
+  else if (t > n) tree[t] = {T(1e10), T(1e10), T(1e10), node_word(T(0), -1)};
+  if (t < n_dev) {
+    if (!heap_order) hash[t] = {(uint32_t)t, (uint32_t)t};
+    else if (t >= n) hash[t] = {1u << 24, (uint32_t)t};
+  }
+}
+
+// level L on the split axis's list (m unplaced particles): place the medians,
+// give every other particle its child subtree, and move it to its run of the
+// next level's layout
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_tree_split(const int* __restrict__ la, int* __restrict__ la_next,
+                                                        int* __restrict__ tag, const V4<T>* __restrict__ pos,
+                                                        V4<T>* __restrict__ tree, KV* __restrict__ hash, int m,
+                                                        int L, int n, int axis, int heap_order) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= m) return;
+  const int id = la[idx];
+  const int t = tag[id];
+  const int p = idx - subtree_start(t, L, n);
+  const int lt = 2 * t <= n ? subtree_size(2 * t, n) : 0;
+  const int placed_before = t - (1 << L);   // level-L nodes left of t
+  if (p < lt) {
+    tag[id] = 2 * t;
+    la_next[idx - placed_before] = id;
+  } else if (p == lt) {
+    const V4<T> q = pos[id];
+    tree[t] = {q.x, q.y, q.z, node_word(T(0), id | (axis << 24))};
+    if (heap_order) hash[t - 1] = {(uint32_t)(t - 1), (uint32_t)id};
+  } else {
+    tag[id] = 2 * t + 1;
+    la_next[idx - placed_before - 1] = id;
+  }
+}
+
+// the other two lists: "goes left" flags for the scan, and where in each list
+// the run's placed particle sits
+__global__ __launch_bounds__(kBlock) void k_tree_flags(const int* __restrict__ lb, const int* __restrict__ lc,
+                                                        const int* __restrict__ tag, int* __restrict__ fb,
+                                                        int* __restrict__ fc, int* __restrict__ at_b,
+                                                        int* __restrict__ at_c, int m, int L) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= m) return;
+  const int next_level = 2 << L;   // tags of the next level are >= 2^(L+1); a placed particle kept t < that
+  const int tb = tag[lb[idx]], tc = tag[lc[idx]];
+  fb[idx] = tb >= next_level && !(tb & 1);
+  fc[idx] = tc >= next_level && !(tc & 1);
+  if (tb < next_level) at_b[tb] = idx;
+  if (tc < next_level) at_c[tc] = idx;
+}
+
+__device__ __forceinline__ void regroup_one(const int* __restrict__ l, int* __restrict__ l_next,
+                                            const int* __restrict__ sc, const int* __restrict__ at,
+                                            const int* __restrict__ tag, int idx, int L, int n) {
+  const int id = l[idx];
+  const int nt = tag[id];
+  if (nt < (2 << L)) return;   // placed at this level
+  const int t = nt >> 1;
+  const int s = subtree_start(t, L, n);
+  const int base = s - (t - (1 << L));   // where subtree 2t starts in the next layout
+  const int lefts = sc[idx] - sc[s];
+  if (!(nt & 1)) {
+    l_next[base + lefts] = id;
+  } else {
+    const int lt = subtree_size(2 * t, n);
+    const int rights = (idx - s) - lefts - (at[t] < idx ? 1 : 0);
+    l_next[base + lt + rights] = id;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_tree_regroup(const int* __restrict__ lb, int* __restrict__ lb_next,
+                                                          const int* __restrict__ sb, const int* __restrict__ at_b,
+                                                          const int* __restrict__ lc, int* __restrict__ lc_next,
+                                                          const int* __restrict__ sc, const int* __restrict__ at_c,
+                                                          const int* __restrict__ tag, int m, int L, int n) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= m) return;
+  regroup_one(lb, lb_next, sb, at_b, tag, idx, L, n);
+  regroup_one(lc, lc_next, sc, at_c, tag, idx, L, n);
+}
+
+}  // namespace
+
+long tree_work_ints(long n) { return 13 * n + 8; }
+
+template <class T>
+void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash, int heap_order, int* work,
+                       KV* kv, KV* kv_alt, int* sort_scratch, int* scan_scratch, hipStream_t s) {
+  if (n <= 0 || n > n_dev) throw std::invalid_argument("tree build: need 0 < n <= n_dev");
+  hipLaunchKernelGGL((k_tree_fixed<T>), dim3(ceil_div(n_dev + 2, kBlock)), dim3(kBlock), 0, s, tree, hash, n, n_dev,
+                     heap_order);
+  CS_CHECK_LAUNCH();
+  int* list[3][2];
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 2; ++k) list[a][k] = work + (size_t)(2 * a + k) * n;
+  // the scans write m + 1 outputs (the total last), so each holds n + 1
+  int* tag = work + 6 * (size_t)n;
+  int* fb = work + 7 * (size_t)n;
+  int* fc = work + 8 * (size_t)n;
+  int* sb = work + 9 * (size_t)n;
+  int* sc = work + 10 * (size_t)n + 1;
+  int* at_b = work + 11 * (size_t)n + 2 - 1;   // indexed by heap node 1..n
+  int* at_c = work + 12 * (size_t)n + 2 - 1;
+  const dim3 g(ceil_div(n, kBlock));
+  // the three presorted lists (FP64 coordinates: low key word, then high)
+  for (int a = 0; a < 3; ++a) {
+    const bool wide = sizeof(T) == 8;
+    hipLaunchKernelGGL((k_tree_keys<T>), g, dim3(kBlock), 0, s, pos, kv, n, a, 0, 1);
+    CS_CHECK_LAUNCH();
+    launch_radix_sort(kv, kv_alt, n, 32, sort_scratch, s);
+    if (wide) {
+      hipLaunchKernelGGL((k_tree_keys<T>), g, dim3(kBlock), 0, s, pos, kv, n, a, 1, 0);
+      CS_CHECK_LAUNCH();
+      launch_radix_sort(kv, kv_alt, n, 32, sort_scratch, s);
+    }
+    hipLaunchKernelGGL(k_tree_ids, g, dim3(kBlock), 0, s, kv, list[a][0], tag, n, a == 0);
+    CS_CHECK_LAUNCH();
+  }
+  int cur[3] = {0, 0, 0};
+  int m = n;
+  for (int L = 0; m > 0; ++L) {
+    const int a = L % 3, b = (L + 1) % 3, c = (L + 2) % 3;
+    const dim3 gm(ceil_div(m, kBlock));
+    hipLaunchKernelGGL((k_tree_split<T>), gm, dim3(kBlock), 0, s, list[a][cur[a]], list[a][cur[a] ^ 1], tag, pos,
+                       tree, hash, m, L, n, a, heap_order);
+    CS_CHECK_LAUNCH();
+    const int placed = (int)std::min<long>(1l << L, (long)n - ((1l << L) - 1));
+    if (m - placed > 0) {
+      hipLaunchKernelGGL(k_tree_flags, gm, dim3(kBlock), 0, s, list[b][cur[b]], list[c][cur[c]], tag, fb, fc, at_b,
+                         at_c, m, L);
+      CS_CHECK_LAUNCH();
+      launch_exclusive_scan(fb, sb, m, 0, scan_scratch, s);
+      launch_exclusive_scan(fc, sc, m, 0, scan_scratch, s);
+      hipLaunchKernelGGL(k_tree_regroup, gm, dim3(kBlock), 0, s, list[b][cur[b]], list[b][cur[b] ^ 1], sb, at_b,
+                         list[c][cur[c]], list[c][cur[c] ^ 1], sc, at_c, tag, m, L, n);
+      CS_CHECK_LAUNCH();
+    }
+    for (int k = 0; k < 3; ++k) cur[k] ^= 1;
+    m -= placed;
+  }
+}
+
+template void launch_tree_build<float>(const F4*, int, int, F4*, KV*, int, int*, KV*, KV*, int*, int*, hipStream_t);
+template void launch_tree_build<double>(const D4*, int, int, D4*, KV*, int, int*, KV*, KV*, int*, int*, hipStream_t);
+
+}  // namespace cudasph

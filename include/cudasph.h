@@ -172,7 +172,11 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
  * form: correctly rounded 1/sqrt in gravity, the CPU restatement's bits; 0
  * is the hardware rsq the reference uses); i_order (per-particle kernels in
  * memory order 0 or kd-tree heap order 1); overlap (gravity on a second
- * stream beside the SPH passes); timing (per-kernel HIP-event timers).
+ * stream beside the SPH passes); timing (per-kernel HIP-event timers);
+ * tree_gpu (1: the kd-tree built on the device, the mixed form's default --
+ * the same tree as the host build when no two particles share a coordinate,
+ * otherwise the same neighbour sets; 0: the host build, bit-identical to
+ * kd_tree.cpp:39-128, the plain and FP64 forms' default).
  * Mixed-form gravity measurement switches, all results within the form's
  * tolerance: grav_split_only (1: split hi/lo differences for every pair),
  * grav_unit_mass (0: the general-mass body even when every mass is equal),
