@@ -32,6 +32,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_FP32_TFLOPS = 157.3
+# the same peak without packed FP32 (v_pk_fma_f32), which this build does not
+# emit: one FP32 FMA per lane per instruction
+PEAK_FP32_UNPACKED_TFLOPS = 78.6
 FLOP_PER_PAIR = 20
 WORKLOAD = "two n=1.5 polytropes (R=1, M=0.5 each), parabolic orbit q=0.5 from separation 5; " \
            "gas+radiation EOS mu=0.6 a=1e-3; 64+-8 neighbours; Plummer eps=0.002"
@@ -381,6 +384,8 @@ def main():
         "roofline": {"bound": "valu", "kernel": "k_gravity_mixed_plummer",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profile("k_gravity"),
+                     "peak_unpacked": PEAK_FP32_UNPACKED_TFLOPS,
+                     "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
                      "per_launch": f"{FLOP_PER_PAIR} flop x {nd}^2 pairs",
                      "launch_ms": g_avg * 1e3},
         "far_field": None if far is None else {
