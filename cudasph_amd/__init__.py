@@ -91,6 +91,8 @@ def lib() -> C.CDLL:
     L.sph_engine_read_state.restype = i
     L.sph_engine_write_state.argtypes = [vp, cp]
     L.sph_engine_write_state.restype = i
+    L.sph_engine_output.argtypes = [vp, cp]
+    L.sph_engine_output.restype = i
     L.sph_state_file_count.argtypes = [cp]
     L.sph_state_file_count.restype = i
     L.sph_abi_set_params.argtypes = [C.POINTER(Params), i]
@@ -318,6 +320,14 @@ class Engine:
     def read_state(self, path: str) -> None:
         if lib().sph_engine_read_state(self._h, str(path).encode()) != 0:
             raise _err("read_state")
+
+    def output(self, directory: str) -> int:
+        """output.cpp:3-15: the numbered snapshot when due, and restart.bin;
+        returns the snapshot number written (0: restart.bin only)"""
+        r = lib().sph_engine_output(self._h, str(directory).encode())
+        if r < 0:
+            raise _err("output")
+        return r
 
 
 def make_engine(form, config: int, n: int | None = None, seed: int = 12345, params: Params | None = None,

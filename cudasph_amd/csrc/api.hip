@@ -47,6 +47,8 @@ struct StateHeader {
   double trelax, dt, omega2;
 };
 
+static_assert(sizeof(StateHeader) <= sizeof(EngineBase::state_header), "state header storage");
+
 bool read_header(FILE* f, StateHeader& h) {
   bool ok = true;
   auto rd = [&](void* p, size_t n) { ok = ok && std::fread(p, n, 1, f) == 1; };
@@ -269,29 +271,65 @@ int sph_engine_read_state(void* e, const char* path) {
     en->set_scalar("dt", h.dt);
     en->set_scalar("global_time", h.global_time);
     en->set_scalar("iteration", h.iteration_number);
+    std::memcpy(en->state_header, &h, sizeof(h));
+    en->has_state_header = true;
     return 0;
   }, -1);
 }
 
-int sph_engine_write_state(void* e, const char* path) {
-  return guard([&] {
-    EngineBase* en = E(e);
-    const sph_params& p = en->params();
-    FILE* f = std::fopen(path, "wb");
-    if (!f) throw std::runtime_error(std::string("cannot open file ") + path);
-    StateHeader h{};
-    h.n_bodies = p.n_bodies;
-    h.t_end = 0;
-    h.iteration_number = (int)en->get_scalar("iteration");
-    h.global_time = en->get_scalar("global_time");
+// write_state.cpp:3-62: the header read last (or, for an engine started
+// from initial conditions, one carrying the parameters) with the fields the
+// run advances taken from the engine
+static void write_state_file(EngineBase* en, const char* path) {
+  const sph_params& p = en->params();
+  StateHeader h{};
+  if (en->has_state_header) {
+    std::memcpy(&h, en->state_header, sizeof(h));
+  } else {
     h.alpha = p.av_alpha;
     h.beta = p.av_beta;
-    h.dt = en->get_scalar("dt");
-    write_header(f, h);
-    if (en->get_scalar("real_bytes") == 8) write_state_from<double>(en, f, p.n_bodies, p.n_dev);
-    else write_state_from<float>(en, f, p.n_bodies, p.n_dev);
-    std::fclose(f);
+  }
+  h.n_bodies = p.n_bodies;
+  h.iteration_number = (int)en->get_scalar("iteration");
+  h.global_time = en->get_scalar("global_time");
+  h.dt = en->get_scalar("dt");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) throw std::runtime_error(std::string("cannot open file ") + path);
+  write_header(f, h);
+  if (en->get_scalar("real_bytes") == 8) write_state_from<double>(en, f, p.n_bodies, p.n_dev);
+  else write_state_from<float>(en, f, p.n_bodies, p.n_dev);
+  const bool ok = std::ferror(f) == 0;
+  std::fclose(f);
+  if (!ok) throw std::runtime_error(std::string("write failed: ") + path);
+}
+
+int sph_engine_write_state(void* e, const char* path) {
+  return guard([&] {
+    write_state_file(E(e), path);
     return 0;
+  }, -1);
+}
+
+// output.cpp:3-15: a numbered snapshot each time global_time passes the next
+// multiple of dt_out (dt_out > 0, from the state file's header), and the
+// restart file every call
+int sph_engine_output(void* e, const char* dir) {
+  return guard([&] {
+    EngineBase* en = E(e);
+    StateHeader h{};
+    if (en->has_state_header) std::memcpy(&h, en->state_header, sizeof(h));
+    const std::string base = std::string(dir && *dir ? dir : ".") + "/";
+    int written = 0;
+    if (en->get_scalar("global_time") > h.snapshot_number * h.dt_out && h.dt_out > 0) {
+      ++h.snapshot_number;
+      std::memcpy(en->state_header, &h, sizeof(h));
+      char name[32];
+      std::snprintf(name, sizeof(name), "out%04d.bin", h.snapshot_number);
+      write_state_file(en, (base + name).c_str());
+      written = h.snapshot_number;
+    }
+    write_state_file(en, (base + "restart.bin").c_str());
+    return written;
   }, -1);
 }
 

@@ -37,6 +37,11 @@ class EngineBase {
   virtual double adjoint_forward(int steps, int ckpt_every) = 0;
   virtual void adjoint_backward(double* gx, double* gv, double* gu) = 0;
   virtual double adjoint_replay(const double* x0, const double* v0, const double* u0) = 0;
+  // the header of the last state file read (read_state.cpp:5-23), kept so
+  // that write_state writes back the fields this engine does not evolve
+  // (t_end, dt_out, snapshot_number, ...); opaque here, laid out by api.hip
+  alignas(8) unsigned char state_header[192] = {};
+  bool has_state_header = false;
 };
 
 EngineBase* make_engine(int form, const sph_params& p, int device);

@@ -222,10 +222,17 @@ int    sph_adjoint_backward(void* e, double* grad_x, double* grad_v, double* gra
 int    sph_adjoint_replay(void* e, const double* x0, const double* v0, const double* u0, double* loss);
 
 /* Reference state file (read_state.cpp:5-67 / write_state.cpp:6-61): 124-byte
- * header + 23 doubles per particle. */
+ * header + 23 doubles per particle.  The engine keeps the header it reads and
+ * writes it back with n_bodies, iteration_number, global_time and dt from the
+ * run (the other fields, t_end, dt_out, snapshot_number, ..., as read). */
 int    sph_state_file_count(const char* path);   /* N from the header, or < 0 */
 int    sph_engine_read_state(void* e, const char* path);
 int    sph_engine_write_state(void* e, const char* path);
+/* CUDAsph::output (output.cpp:3-15) into directory `dir`: when global_time >
+ * snapshot_number * dt_out and dt_out > 0, snapshot_number += 1 and
+ * out%04d.bin is written; restart.bin is written every call.  Returns the
+ * snapshot number written, 0 if only restart.bin, -1 on error. */
+int    sph_engine_output(void* e, const char* dir);
 
 /* Copies `bytes` between two fresh device buffers twice with a plain copy
  * kernel (k_copy16) and returns the milliseconds of the second copy: the

@@ -43,6 +43,8 @@ def make_file(path, n, seed):
     h = np.zeros(1, HEADER)
     h["n_bodies"], h["nnopt"], h["snapshot_number"], h["iteration_number"] = n, 58, 7, 1234
     h["global_time"], h["dt"], h["alpha"], h["beta"] = 3.25, 0.0078125, 1.0, 2.0
+    h["hmin"], h["hmax"], h["sep0"], h["t_end"], h["dt_out"] = 1e-3, 0.5, 5.0, 10.0, 0.4
+    h["nav"], h["eta2"], h["ngr"], h["nrelax"], h["trelax"], h["omega2"] = 3, 0.01, 1, 0, 2.5, 0.125
     r = np.zeros(n, RECORD)
     for name in RECORD.names:
         # float32-representable values, so every form holds them exactly
@@ -106,4 +108,42 @@ def test_state_file_of_another_size_is_refused(cs, tmp_path):
     e = cs.Engine("plain", cs.preset_params(1, 300))
     with pytest.raises(RuntimeError, match="differs"):
         e.read_state(src)
+    e.close()
+
+
+@pytest.mark.gpu
+def test_output_and_header_follow_the_reference(cs, tmp_path):
+    """output.cpp:3-15 with dt_out = 0.4, snapshot_number = 7 and
+    global_time = 3.25: 3.25 > 7 * 0.4 and then > 8 * 0.4, so two calls write
+    out0008.bin and out0009.bin, a third only restart.bin.  Every header field
+    the run does not advance is written back as read."""
+    n = 300
+    print(f"state file output n={n}", flush=True)
+    src = tmp_path / "in.bin"
+    h, r = make_file(src, n, seed=5)
+    e = cs.Engine("plain", cs.preset_params(1, n))
+    e.read_state(src)
+    out = tmp_path / "run"
+    out.mkdir()
+    assert e.output(out) == 8
+    assert e.output(out) == 9
+    assert e.output(out) == 0
+    assert sorted(p.name for p in out.iterdir()) == ["out0008.bin", "out0009.bin", "restart.bin"]
+    for name, snap in (("out0008.bin", 8), ("out0009.bin", 9), ("restart.bin", 9)):
+        raw = (out / name).read_bytes()
+        h2 = np.frombuffer(raw[:HEADER.itemsize], HEADER)[0]
+        for field in HEADER.names:
+            want = snap if field == "snapshot_number" else h[field]
+            assert h2[field] == want, (name, field)
+        assert np.array_equal(np.frombuffer(raw[HEADER.itemsize:], RECORD)["x"], r["x"])
+    e.close()
+
+
+@pytest.mark.gpu
+def test_output_without_a_state_file_writes_the_restart_only(cs, tmp_path):
+    e = cs.make_engine("plain", 1, 2000, 3)
+    assert e.output(tmp_path) == 0
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["restart.bin"]
+    h2 = np.frombuffer((tmp_path / "restart.bin").read_bytes()[:HEADER.itemsize], HEADER)[0]
+    assert h2["n_bodies"] == 2000 and h2["alpha"] == 1.0 and h2["beta"] == 2.0
     e.close()
