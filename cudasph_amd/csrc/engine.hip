@@ -243,6 +243,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_expand") return grav_expand_;
   if (name == "grav_far") return grav_far_;
   if (name == "grav_far_ratio") return grav_far_ratio_;
+  if (name == "grav_far_grade") return grav_far_grade_;
   if (name == "overlap") return overlap_;
   if (name == "tree_gpu") return tree_gpu_;
   if (name == "timing") return timers_.enabled;
@@ -277,6 +278,9 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
   } else if (name == "grav_far_ratio") {
     if (!(v >= 4)) throw std::invalid_argument("grav_far_ratio must be at least 4");
     grav_far_ratio_ = v;
+  } else if (name == "grav_far_grade") {
+    if (!(v >= 0 && v <= 0.2)) throw std::invalid_argument("grav_far_grade must lie in [0, 0.2]");
+    grav_far_grade_ = v;
   } else if (name == "overlap") {
     overlap_ = (int)v;
   } else if (name == "tree_gpu") {
@@ -558,7 +562,7 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
   TimerScope ts(timers_, "gravity", gs);
   if constexpr (kF32) {
     if (mixed()) {
-      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, gmass_, gmom_, (float)grav_far_ratio_, nd_, nd_,
+      launch_gravity_mixed(ghi_, glo_, gvel32_, (F4*)grav_.p, gpart_, gmass_, gmom_, (float)grav_far_ratio_, (float)grav_far_grade_, nd_, nd_,
                            n_, p_.grav_mode,
                            p_.grav_eps,
                            grav_split_only_ | (grav_unit_mass_ ? 0 : 2) | (grav_expand_ ? 0 : 4) |

@@ -193,7 +193,8 @@ class Engine final : public EngineBase {
   int grav_unit_mass_ = 1;    // mixed gravity: unit-mass pair body when every mass is equal
   int grav_expand_ = 1;       // mixed gravity: expanded r^2 in the equal-mass tile body
   int grav_far_ = 0;          // mixed gravity: order-4 multipole far field for distant sub-tiles
-  double grav_far_ratio_ = 8;  // ... taken beyond this many sub-tile radii
+  double grav_far_ratio_ = 8;  // ... taken beyond this many radii of a 16384-record cell
+  double grav_far_grade_ = 0;  // ... the ratio at a level of C records scaled by (C / 16384)^grade
   int i_order_ = 1;   // 1: lanes take particles in kd-tree heap order, 0: in memory order
   // 1: the kd-tree is built on the device (same tree without coordinate
   // ties; with ties the same neighbour sets; the mixed form's default);

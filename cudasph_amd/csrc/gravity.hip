@@ -327,6 +327,11 @@ __global__ __launch_bounds__(kGravBlock) void k_subtile_moments(const F4* __rest
 // would round at ~1e-5.
 constexpr int kFarLevels = 4;
 constexpr int kCellSize[kFarLevels] = {16384, 4096, 1024, 256};   // coarse to fine
+// squared acceptance ratios: [L] for the cell level L, [kFarLevels] for
+// the 64-record sub-tiles
+struct FarRatios {
+  float q2[kFarLevels + 1];
+};
 
 __device__ __forceinline__ double block_sum(double v, double* s_red) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -528,7 +533,7 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
                                                                        int split_only,
                                                                        const float* __restrict__ mass_range,
                                                                        const float* __restrict__ mom,
-                                                                       float far_q2,
+                                                                       FarRatios far_q2,
                                                                        const float* __restrict__ cells,
                                                                        int n_real) {
   // every mass equal (mass_range = {min, max} over the real particles): the
@@ -587,7 +592,7 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
             const float yx = ((pi.x - M[48]) + li.x) - M[0], yy = ((pi.y - M[49]) + li.y) - M[1],
                         yz = ((pi.z - M[50]) + li.z) - M[2];
             const float y2 = __builtin_fmaf(yx, yx, __builtin_fmaf(yy, yy, yz * yz));
-            if (M[4] == 0.f || __all(!real_i || y2 >= far_q2 * M[3])) {
+            if (M[4] == 0.f || __all(!real_i || y2 >= far_q2.q2[L] * M[3])) {
               Mc = M;
               Cc = C;
             }
@@ -673,7 +678,7 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
         const float yx = ((pi.x - M[48]) + li.x) - M[0], yy = ((pi.y - M[49]) + li.y) - M[1],
                     yz = ((pi.z - M[50]) + li.z) - M[2];
         const float y2 = __builtin_fmaf(yx, yx, __builtin_fmaf(yy, yy, yz * yz));
-        if (M[4] == 0.f || __all(!real_i || y2 >= far_q2 * M[3])) {
+        if (M[4] == 0.f || __all(!real_i || y2 >= far_q2.q2[kFarLevels] * M[3])) {
           if (M[4] != 0.f) add_far(M, yx, yy, yz);
           if ((sub & 1) || j0 + kTile >= cnt)
             for (int k = 0; k < 4; ++k) a[k].tile_end();
@@ -891,8 +896,8 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
 }
 
 void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, float* mass_range,
-                          float* moments, float far_ratio, int n_i, int n_j, int n_real, int soft, float eps,
-                          int split_only, hipStream_t s) {
+                          float* moments, float far_ratio, float far_grade, int n_i, int n_j, int n_real,
+                          int soft, float eps, int split_only, hipStream_t s) {
   const dim3 grid(ceil_div(n_i, kGravBlock)), block(kGravBlock);
   if (soft == 0) {
     hipLaunchKernelGGL(k_gravity_mixed_spline, grid, block, 0, s, hi, lo, gvel, out, n_i, n_j);
@@ -923,14 +928,24 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
         off += (size_t)n_cells * kMomWords;
       }
     }
+    // ratio q at the coarsest level, graded by (C / C_coarsest)^far_grade:
+    // an order-4 group's truncation error goes as M R^5 / y^7, and with
+    // equal masses M grows with the record count C while R grows as
+    // C^(1/3), so a grade near 1/15 asks every level for the same bound
+    FarRatios fq{};
+    for (int L = 0; L <= kFarLevels; ++L) {
+      const float c = (float)(L < kFarLevels ? kCellSize[L] : kTile) / (float)kCellSize[0];
+      const float q = far_ratio * powf(c, far_grade);
+      fq.q2[L] = q * q;
+    }
     if ((split_only & 16) && moments)
       hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, true>), dim3(ceil_div(n_i, kGravBlock) * kGravParts),
                          block, 0, s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range,
-                         moments, far_ratio * far_ratio, cells, n_real);
+                         moments, fq, cells, n_real);
     else
       hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, false>), dim3(ceil_div(n_i, kGravBlock) * kGravParts),
                          block, 0, s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range,
-                         nullptr, 0.f, nullptr, n_real);
+                         nullptr, fq, nullptr, n_real);
     CS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gravity_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part_hi, part_lo, out,
                        n_i, mass_range, split_only);

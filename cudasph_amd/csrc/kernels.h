@@ -54,10 +54,11 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
 // moments: scratch of 52 floats per 64-record j sub-tile followed by 52 per
 // cell of 16384, 4096, 1024 and 256 records (the optional multipole far
 // field, split_only bit 4, taken where every lane of a wavefront is
-// far_ratio cell radii away; may be null otherwise)
+// far_ratio radii away, the ratio at a level of C records scaled by
+// (C / 16384)^far_grade; may be null otherwise)
 void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, float* mass_range,
-                          float* moments, float far_ratio, int n_i, int n_j, int n_real, int soft, float eps,
-                          int split_only, hipStream_t s);
+                          float* moments, float far_ratio, float far_grade, int n_i, int n_j, int n_real,
+                          int soft, float eps, int split_only, hipStream_t s);
 
 // ---- sort.hip ----
 // exclusive scan of in[0..n) into out[0..n]; out[n] = total.  mode 1 scans
