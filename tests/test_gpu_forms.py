@@ -331,15 +331,18 @@ def test_mixed_far_field_trajectory_against_fp64(cs, orc):
     assert np.abs(x - xo).max() <= 1e-6 * np.abs(xo).max()
 
 
-@pytest.mark.parametrize("form", ["plain", "mixed"])
-def test_gravity_stream_overlap_changes_nothing(cs, form):
+@pytest.mark.parametrize("form,far", [("plain", 0), ("mixed", 0), ("mixed", 1)])
+def test_gravity_stream_overlap_changes_nothing(cs, form, far):
     """Gravity on its own stream beside the SPH passes gives the same bits as
-    the serial order."""
+    the serial order (with the far field, the default overlap -1 takes the
+    second stream)."""
     n = 16384
     out = []
-    for ov in (0, 1):
+    for ov in (0, 1 if not far else -1):
         e = cs.make_engine(form, 3, n, 9, ph_sort=True)
         e.set_scalar("overlap", ov)
+        if far:
+            e.set_scalar("grav_far", 1)
         e.init()
         e.step(3)
         out.append({k: e.get(k)[:n].copy() for k in ("pos", "vel", "hydro", "bdots", "grav")})
