@@ -707,11 +707,12 @@ void Engine<T>::step() {
   // softening needs the new ranges and keeps that order.
   const bool grav_first = p_.grav_mode == 1;
   tree_fetch();            // also reads back dt_
-  // overlap < 0 (the default): on for the far-field gravity, whose kernel
-  // runs at 5 wavefronts per SIMD and leaves room for the SPH passes
-  // (106.2 -> 101.6 ms/step at N = 2^20), off for the pairwise kernel, which
-  // they slowed by more than the time they hid
-  const bool overlap = overlap_ >= 0 ? overlap_ != 0 : (mixed() && grav_far_ != 0);
+  // overlap < 0 (the default): on for the mixed form, whose SPH passes take
+  // ~12 ms at N = 2^20 and slow its gravity by ~4 ms when they run beside it
+  // (288.5 -> 282.9 ms/step; with the far field 106.2 -> 101.6), off for the
+  // plain and FP64 forms, whose longer h-solve walks lose more than they hide
+  // (plain: 1057 -> 1215 ms/step)
+  const bool overlap = overlap_ >= 0 ? overlap_ != 0 : mixed();
   if (grav_first) gravity(overlap);
   tree_build();
   sph_after_tree();

@@ -186,9 +186,7 @@ class Engine final : public EngineBase {
   double dt_ = 0, global_time_ = 0;
   int iteration_ = 0, exact_rsqrt_ = 0;
   // 1: gravity on gs_ beside the SPH passes (Plummer softening), 0: in
-  // order, -1 (default): only with the mixed form's far field.  Measured at
-  // N = 2^20 the co-running SPH kernels slowed the pairwise gravity by more
-  // than the SPH time they hid, and sped up the far-field step by 4 %
+  // order, -1 (default): in the mixed form only (measured in Engine::step)
   int overlap_ = -1;
   int grav_split_only_ = 0;
   int grav_unit_mass_ = 1;    // mixed gravity: unit-mass pair body when every mass is equal

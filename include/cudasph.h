@@ -172,8 +172,8 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
  * form: correctly rounded 1/sqrt in gravity, the CPU restatement's bits; 0
  * is the hardware rsq the reference uses); i_order (per-particle kernels in
  * memory order 0 or kd-tree heap order 1); overlap (1: gravity on a second
- * stream beside the SPH passes, 0: in order, -1, the default: only with the
- * mixed form's far field; the results are the same bits either way);
+ * stream beside the SPH passes, 0: in order, -1, the default: in the mixed
+ * form only; the results are the same bits either way);
  * timing (per-kernel HIP-event timers);
  * tree_gpu (1: the kd-tree built on the device, the mixed form's default --
  * the same tree as the host build when no two particles share a coordinate,
