@@ -40,6 +40,11 @@ WORKLOAD = "two n=1.5 polytropes (R=1, M=0.5 each), parabolic orbit q=0.5 from s
            "gas+radiation EOS mu=0.6 a=1e-3; 64+-8 neighbours; Plummer eps=0.002"
 
 
+def progress(msg):
+    """one line on stderr per finished leg (stdout carries only the JSON line)"""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_form(cs, form, n, seed, warmup, steps, timing=True, i_order=0, scalars=None):
     # the initial conditions are put in Peano-Hilbert order (the reference's
     # periodic sort_bodies) so neighbour gathers read nearby memory
@@ -78,6 +83,7 @@ def run_form(cs, form, n, seed, warmup, steps, timing=True, i_order=0, scalars=N
         # in order, after everything above was taken, time each pass alone
         out["timers_in_order"] = timers_in_order(e, 2)
     e.close()
+    progress(f"{form}{' ' + str(scalars) if scalars else ''}: {steps} steps in {wall:.2f} s")
     return out
 
 
