@@ -419,8 +419,8 @@ def main():
                      "launch_ms": g_avg * 1e3},
         "far_field": None if far is None else {
             "what": "mixed form with the order-4 multipole far field (grav_far = 1): per wavefront, "
-                    "the coarsest cell of 16384/4096/1024/256 records or 64-record sub-tile seen from "
-                    "beyond 8 of its radii takes one expansion; the rest unchanged",
+                    "the coarsest cell of 16384/4096/1024/256 records, 64-record sub-tile or 32-record "
+                    "half seen from beyond 8 of its radii takes one expansion; the rest unchanged",
             "value": n * K / far["wall_s"], "unit": "particle-updates/s", "ms_per_step": far["wall_s"] / K * 1e3,
             "gravity_ms": far["timers"].get("gravity", (0, 1))[0] / max(far["timers"].get("gravity", (0, 1))[1], 1),
             "over_pairwise_mixed": mixed["wall_s"] / far["wall_s"],

@@ -123,9 +123,10 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
     gvel32_.alloc(nd_);
     gpart_.alloc(2 * 8 * (size_t)nd_);
     gmass_.alloc(2 + 2 * (size_t)ceil_div(n_, kBlock));
-    // far-field records: 64-record sub-tiles, then cells of 16384, 4096, 1024, 256
+    // far-field records: 64-record sub-tiles, then cells of 16384, 4096, 1024, 256,
+    // then the sub-tiles' 32-record halves
     gmom_.alloc(52 * ((size_t)ceil_div(nd_, 64) + ceil_div(nd_, 16384) + ceil_div(nd_, 4096) + ceil_div(nd_, 1024) +
-                      ceil_div(nd_, 256)));
+                      ceil_div(nd_, 256) + 2 * (size_t)ceil_div(nd_, 64)));
     CS_HIP(hipMemsetAsync(pos_lo_, 0, pos_lo_.bytes(), s_));
     CS_HIP(hipMemsetAsync(pos64_, 0, pos64_.bytes(), s_));
     CS_HIP(hipMemsetAsync(vel64_, 0, vel64_.bytes(), s_));
@@ -244,6 +245,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_far") return grav_far_;
   if (name == "grav_far_ratio") return grav_far_ratio_;
   if (name == "grav_far_grade") return grav_far_grade_;
+  if (name == "grav_far_half") return grav_far_half_;
   if (name == "overlap") return overlap_;
   if (name == "tree_gpu") return tree_gpu_;
   if (name == "timing") return timers_.enabled;
@@ -278,6 +280,8 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
   } else if (name == "grav_far_ratio") {
     if (!(v >= 4)) throw std::invalid_argument("grav_far_ratio must be at least 4");
     grav_far_ratio_ = v;
+  } else if (name == "grav_far_half") {
+    grav_far_half_ = (int)v;
   } else if (name == "grav_far_grade") {
     if (!(v >= 0 && v <= 0.2)) throw std::invalid_argument("grav_far_grade must lie in [0, 0.2]");
     grav_far_grade_ = v;
@@ -566,7 +570,7 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
                            n_, p_.grav_mode,
                            p_.grav_eps,
                            grav_split_only_ | (grav_unit_mass_ ? 0 : 2) | (grav_expand_ ? 0 : 4) |
-                               (grav_far_ ? 16 : 0),
+                               (grav_far_ ? 16 : 0) | (grav_far_ && grav_far_half_ ? 64 : 0),
                            gs);
       return;
     }

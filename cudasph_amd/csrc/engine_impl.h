@@ -193,6 +193,7 @@ class Engine final : public EngineBase {
   int grav_expand_ = 1;       // mixed gravity: expanded r^2 in the equal-mass tile body
   int grav_far_ = 0;          // mixed gravity: order-4 multipole far field for distant sub-tiles
   double grav_far_ratio_ = 8;  // ... taken beyond this many radii of a 16384-record cell
+  int grav_far_half_ = 1;      // ... also the 32-record halves of a sub-tile not taken whole
   double grav_far_grade_ = 0;  // ... the ratio at a level of C records scaled by (C / 16384)^grade
   int i_order_ = 1;   // 1: lanes take particles in kd-tree heap order, 0: in memory order
   // 1: the kd-tree is built on the device (same tree without coordinate

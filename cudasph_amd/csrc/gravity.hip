@@ -18,6 +18,8 @@
 //   EXACT 1: 1/sqrt correctly rounded (bit-exact with the CPU restatement);
 //         0: hardware reciprocal square root (the reference uses rsqrtf)
 //   HILO  positions carry low words (mixed form): r_ij = (hi_i-hi_j)+(lo_i-lo_j)
+#include <type_traits>
+
 #include "kernels.h"
 #include "sph_math.h"
 
@@ -250,16 +252,18 @@ __device__ __forceinline__ float wave_sum(float v) {
 // padding records (j >= n_real) are left out.  One wavefront per sub-tile.
 __global__ __launch_bounds__(kGravBlock) void k_subtile_moments(const F4* __restrict__ hi, const F4* __restrict__ lo,
                                                                 float* __restrict__ mom, int n_j, int n_real,
-                                                                int n_sub, int flags,
+                                                                int n_sub, int G, int flags,
                                                                 const float* __restrict__ mass_range) {
   const bool uniform = (flags & 2) == 0 && mass_range[0] == mass_range[1];
   const int s = blockIdx.x * (kGravBlock / 64) + (int)(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (s >= n_sub) return;   // whole wavefronts
-  const int j0 = s * kTile, nj = min(kTile, n_j - j0);
+  // a group of G <= 64 records; lanes past it add nothing (a group past
+  // n_j, possible for G < 64, is empty: M0 = 0, skipped where it is read)
+  const int j0 = s * G, nj = min(G, n_j - j0);
   // origin: the middle real record (a padding record sits at 1e30)
   const int nr = max(0, min(nj, n_real - j0));
-  const F4 c = hi[j0 + nr / 2];
+  const F4 c = hi[min(j0 + nr / 2, n_j - 1)];
   const int j = j0 + lane;
   const bool valid = lane < nj && j < n_real;
   float dx = 0.f, dy = 0.f, dz = 0.f, m = 0.f;
@@ -328,9 +332,9 @@ __global__ __launch_bounds__(kGravBlock) void k_subtile_moments(const F4* __rest
 constexpr int kFarLevels = 4;
 constexpr int kCellSize[kFarLevels] = {16384, 4096, 1024, 256};   // coarse to fine
 // squared acceptance ratios: [L] for the cell level L, [kFarLevels] for
-// the 64-record sub-tiles
+// the 64-record sub-tiles, [kFarLevels + 1] for their 32-record halves
 struct FarRatios {
-  float q2[kFarLevels + 1];
+  float q2[kFarLevels + 2];
 };
 
 __device__ __forceinline__ double block_sum(double v, double* s_red) {
@@ -535,6 +539,7 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
                                                                        const float* __restrict__ mom,
                                                                        FarRatios far_q2,
                                                                        const float* __restrict__ cells,
+                                                                       const float* __restrict__ halves,
                                                                        int n_real) {
   // every mass equal (mass_range = {min, max} over the real particles): the
   // pair body drops its mass factor and k_gravity_combine scales by m once;
@@ -545,6 +550,8 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
   // a separate instantiation: the far-field code raises the VGPR count, which
   // would lower the occupancy of the pairwise kernel
   const bool far_on = FAR && (split_only & 1) == 0;
+  // split_only bit 6: 32-record halves of a sub-tile not taken whole
+  const bool half_on = far_on && (split_only & 64) != 0 && halves != nullptr;
   split_only &= 1;
   constexpr int kSub = JT / kTile;   // sub-tiles of kTile records, each with its own origin
   __shared__ F4 s_hi[JT], s_lo[JT], s_rel[JT];
@@ -666,6 +673,7 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
     };
     for (int sub = 0; sub * kTile < cnt; ++sub) {
       const int j0 = sub * kTile, nj = min(kTile, cnt - j0);
+      int jb = j0, je = j0 + nj;   // the records taken pair by pair
       const F4 c = s_c[sub];
       const float dx = (pi.x - c.x) + li.x, dy = (pi.y - c.y) + li.y, dz = (pi.z - c.z) + li.z;
       // tile form only if every lane of the wavefront has |d_i| >= 3 R_sub
@@ -683,6 +691,26 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
           if ((sub & 1) || j0 + kTile >= cnt)
             for (int k = 0; k < 4; ++k) a[k].tile_end();
           continue;
+        }
+        if (half_on && nj == kTile) {
+          // each 32-record half the wavefront sees from beyond q of its radii
+          // takes its expansion; the pairs run over the rest
+          bool took[2];
+          for (int h = 0; h < 2; ++h) {
+            const float* H = halves + ((size_t)(t0 / kTile + sub) * 2 + h) * kMomWords;
+            const float hx = ((pi.x - H[48]) + li.x) - H[0], hy = ((pi.y - H[49]) + li.y) - H[1],
+                        hz = ((pi.z - H[50]) + li.z) - H[2];
+            const float h2 = __builtin_fmaf(hx, hx, __builtin_fmaf(hy, hy, hz * hz));
+            took[h] = H[4] == 0.f || __all(!real_i || h2 >= far_q2.q2[kFarLevels + 1] * H[3]);
+            if (took[h] && H[4] != 0.f) add_far(H, hx, hy, hz);
+          }
+          if (took[0] && took[1]) {
+            if ((sub & 1) || j0 + kTile >= cnt)
+              for (int k = 0; k < 4; ++k) a[k].tile_end();
+            continue;
+          }
+          if (took[0]) jb = j0 + kTile / 2;
+          if (took[1]) je = j0 + kTile / 2;
         }
       }
       const bool far = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) >= 9.9f * s_r2[sub];
@@ -706,20 +734,25 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
       // full sub-tiles (all of them when n_j is a multiple of kTile) have a
       // fixed trip count, unrolled so LDS reads are issued ahead of their use;
       // the branches are uniform per wavefront
-      if (nj == kTile) {
+      // the pairs of `len` consecutive staged records from `base`: a whole
+      // sub-tile (len 64) or, in the far-field kernel, the half the
+      // wavefront did not take by its expansion (len 32); fixed trip counts,
+      // unrolled so LDS reads are issued ahead of their use
+      auto run = [&](int base, auto len_c) {
+        constexpr int len = decltype(len_c)::value;
         if (uniform) {
           if (tile_form && expand) {
             // expanded equal-mass body: r^2 + eps^2 = (|d_i|^2 + eps^2 + |d_j|^2)
             // - 2 d_i.d_j and sum_j s3 (d_j - d_i) = M - d_i S over the
-            // sub-tile, 11 VALU + rsq per pair.  With |d_j| <= |d_i| / 3 the
+            // records, 11 VALU + rsq per pair.  With |d_j| <= |d_i| / 3 the
             // expansion loses at most a factor (4/2)^2 of relative accuracy
             // in r^2 to cancellation, and M - d_i S none (|M| <= |d_i| S / 3).
             const float ci = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
             const float bx = -2.f * dx, by = -2.f * dy, bz = -2.f * dz;
             float sa = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
 #pragma unroll 16
-            for (int j = 0; j < kTile; ++j) {
-              const F4 pj = s_rel[j0 + j];
+            for (int j = 0; j < len; ++j) {
+              const F4 pj = s_rel[base + j];
               const float d2 =
                   __builtin_fmaf(pj.x, bx, __builtin_fmaf(pj.y, by, __builtin_fmaf(pj.z, bz, ci + pj.w)));
               const float inv = __builtin_amdgcn_rsqf(d2);
@@ -735,18 +768,30 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
             a[2].part += __builtin_fmaf(-dz, sa, mz);
           } else if (tile_form) {
 #pragma unroll 16
-            for (int j = 0; j < kTile; ++j) unit_tile(j0 + j);
+            for (int j = 0; j < len; ++j) unit_tile(base + j);
           } else {
 #pragma unroll 16
-            for (int j = 0; j < kTile; ++j) unit_split(j0 + j);
+            for (int j = 0; j < len; ++j) unit_split(base + j);
           }
         } else if (tile_form) {
 #pragma unroll 16
-          for (int j = 0; j < kTile; ++j) pair_tile(j0 + j);
+          for (int j = 0; j < len; ++j) pair_tile(base + j);
         } else {
 #pragma unroll 16
-          for (int j = 0; j < kTile; ++j) pair_split(j0 + j);
+          for (int j = 0; j < len; ++j) pair_split(base + j);
         }
+      };
+      // full sub-tiles (all of them when n_j is a multiple of kTile); the
+      // branches are uniform per wavefront
+      if (nj == kTile) {
+        bool whole = true;
+        if constexpr (FAR) {
+          if (je - jb != kTile) {
+            run(jb, std::integral_constant<int, kTile / 2>{});
+            whole = false;
+          }
+        }
+        if (whole) run(j0, std::integral_constant<int, kTile>{});
       } else {
         for (int j = j0; j < j0 + nj; ++j) {
           if (uniform) {
@@ -912,10 +957,11 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
     F4* part_hi = part;
     F4* part_lo = part + (size_t)kGravParts * n_i;
     float* cells = nullptr;
+    float* halves = nullptr;
     if ((split_only & 16) && moments) {
       const int n_sub = ceil_div(n_j, kTile);
       hipLaunchKernelGGL(k_subtile_moments, dim3(ceil_div(n_sub, kGravBlock / 64)), block, 0, s, hi, lo, moments,
-                         n_j, n_real, n_sub, split_only, mass_range);
+                         n_j, n_real, n_sub, kTile, split_only, mass_range);
       CS_CHECK_LAUNCH();
       // the cell levels follow the sub-tile records in the same buffer
       cells = moments + (size_t)n_sub * kMomWords;
@@ -927,25 +973,33 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
         CS_CHECK_LAUNCH();
         off += (size_t)n_cells * kMomWords;
       }
+      if (split_only & 64) {   // the 32-record halves of the sub-tiles, after the cells
+        halves = cells + off;
+        const int n_half = 2 * n_sub;
+        hipLaunchKernelGGL(k_subtile_moments, dim3(ceil_div(n_half, kGravBlock / 64)), block, 0, s, hi, lo, halves,
+                           n_j, n_real, n_half, kTile / 2, split_only, mass_range);
+        CS_CHECK_LAUNCH();
+      }
     }
     // ratio q at the coarsest level, graded by (C / C_coarsest)^far_grade:
     // an order-4 group's truncation error goes as M R^5 / y^7, and with
     // equal masses M grows with the record count C while R grows as
     // C^(1/3), so a grade near 1/15 asks every level for the same bound
     FarRatios fq{};
-    for (int L = 0; L <= kFarLevels; ++L) {
-      const float c = (float)(L < kFarLevels ? kCellSize[L] : kTile) / (float)kCellSize[0];
+    for (int L = 0; L <= kFarLevels + 1; ++L) {
+      const float c = (float)(L < kFarLevels ? kCellSize[L] : (L == kFarLevels ? kTile : kTile / 2)) /
+                      (float)kCellSize[0];
       const float q = far_ratio * powf(c, far_grade);
       fq.q2[L] = q * q;
     }
     if ((split_only & 16) && moments)
       hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, true>), dim3(ceil_div(n_i, kGravBlock) * kGravParts),
                          block, 0, s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range,
-                         moments, fq, cells, n_real);
+                         moments, fq, cells, halves, n_real);
     else
       hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, false>), dim3(ceil_div(n_i, kGravBlock) * kGravParts),
                          block, 0, s, hi, lo, part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range,
-                         nullptr, fq, nullptr, n_real);
+                         nullptr, fq, nullptr, nullptr, n_real);
     CS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gravity_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part_hi, part_lo, out,
                        n_i, mass_range, split_only);

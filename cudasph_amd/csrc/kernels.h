@@ -52,7 +52,8 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
 // mass_range: scratch of 2 + 2 * ceil(n_real / 256) floats ({min, max} of the
 // real particles' masses, then per-block partials)
 // moments: scratch of 52 floats per 64-record j sub-tile followed by 52 per
-// cell of 16384, 4096, 1024 and 256 records (the optional multipole far
+// cell of 16384, 4096, 1024 and 256 records and, with split_only bit 6, 52
+// per 32-record half sub-tile (the optional multipole far
 // field, split_only bit 4, taken where every lane of a wavefront is
 // far_ratio radii away, the ratio at a level of C records scaled by
 // (C / 16384)^far_grade; may be null otherwise)

@@ -186,8 +186,10 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
  * Mixed-form gravity options: grav_far (1: an order-4 multipole far field for
  * each cell of 16384, 4096, 1024 or 256 records, or 64-record sub-tile, that a
  * whole wavefront sees from beyond grav_far_ratio of its radii, default 8,
- * coarsest first; not pairwise, so off by default); grav_far_grade (the
- * ratio at a level of C records scaled by (C / 16384)^grade, default 0).
+ * coarsest first; not pairwise, so off by default); grav_far_half (1, the
+ * default: a sub-tile not taken whole is tried as two 32-record halves);
+ * grav_far_grade (the ratio at a level of C records scaled by
+ * (C / 16384)^grade, default 0).
  * Read only: ngb_tot, overflow_reruns, cap, real_bytes, form.  Unknown names
  * return NaN (get) or -1 with sph_last_error() (set). */
 double sph_engine_get_scalar(void* e, const char* name);

@@ -253,8 +253,9 @@ def test_mixed_gravity_against_fp64(cs, orc, equal):
 @pytest.mark.parametrize("cfg,n,equal", [(3, 16384, True), (3, 16384, False), (3, 16000, True),
                                          (5, 262144, True)])
 def test_mixed_gravity_far_field_against_fp64(cs, orc, cfg, n, equal):
-    """The optional multipole far field (grav_far = 1: order-4 moments of each
-    64-record sub-tile for wavefronts 8 sub-tile radii away) against FP64
+    """The optional multipole far field (grav_far = 1: order-4 moments of cells,
+    64-record sub-tiles and, by default, their 32-record halves, for
+    wavefronts 8 group radii away), with and without the halves, against FP64
     gravity on the same positions, to the pairwise kernel's tolerance; equal
     masses (unit-mass moments) and masses spread over a factor of 3.  The
     uniform sphere needs N = 262144 for its sub-tiles to be small enough
@@ -275,9 +276,15 @@ def test_mixed_gravity_far_field_against_fp64(cs, orc, cfg, n, equal):
     e.run_pass(cs.GRAVITY)
     g_direct = e.get("grav")[:n].astype(np.float64)
     e.set_scalar("grav_far", 1)
+    e.set_scalar("grav_far_half", 0)
+    e.run_pass(cs.GRAVITY)
+    g_whole = e.get("grav")[:n].astype(np.float64)
+    e.set_scalar("grav_far_half", 1)   # the default: also 32-record halves
     e.run_pass(cs.GRAVITY)
     g = e.get("grav")[:n].astype(np.float64)
-    assert not np.array_equal(g, g_direct), "the far field was not taken"
+    assert not np.array_equal(g_whole, g_direct), "the far field was not taken"
+    if n >= 262144:
+        assert not np.array_equal(g, g_whole), "no 32-record half was taken"
     assert np.isfinite(e.get("grav")).all(), "padding records must stay finite"
     if n <= 16384:
         o = orc.Oracle(64, e.params.copy())
@@ -301,8 +308,10 @@ def test_mixed_gravity_far_field_against_fp64(cs, orc, cfg, n, equal):
         go = f.get("grav")[:n]
         f.close()
     rms = np.sqrt((go[:, :3] ** 2).sum(1).mean())
-    err = np.linalg.norm(g[:, :3] - go[:, :3], axis=1) / np.maximum(np.linalg.norm(go[:, :3], axis=1), 1e-3 * rms)
-    assert err.max() < 1e-5, err.max()
+    for res in (g_whole, g):
+        err = np.linalg.norm(res[:, :3] - go[:, :3], axis=1) / np.maximum(np.linalg.norm(go[:, :3], axis=1),
+                                                                         1e-3 * rms)
+        assert err.max() < 1e-5, err.max()
     np.testing.assert_allclose(g[:, 3], go[:, 3], rtol=2e-6)
 
 
