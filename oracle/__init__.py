@@ -1,8 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, and only as the checker or the timed CPU
-baseline.  The product package (cudasph_amd) never imports it.
+cpu_baseline leg of bench.py, and only as the reference the results are
+compared with or as the timed CPU baseline.  The product package (cudasph_amd) never imports it.
 """
 from __future__ import annotations
 

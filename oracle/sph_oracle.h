@@ -2,8 +2,8 @@
  * CPU oracle for the SPH hot path of 3gx/CUDAsph -- C API.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py load liboracle.so, and only as the checker or
- * as the timed CPU baseline.  The product library (cudasph_amd/) never links,
+ * cpu_baseline leg of bench.py load liboracle.so, and only as the reference
+ * the results are compared with or as the timed CPU baseline.  The product library (cudasph_amd/) never links,
  * loads or calls anything in oracle/.
  *
  * The parameter block has the same field layout as sph_params in

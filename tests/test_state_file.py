@@ -1,4 +1,4 @@
-"""The reference's binary state file (survey row f1), field by field.
+"""The reference's binary state file, field by field.
 
 The layouts below restate read_state.cpp:5-23 (the 124-byte header: ints and
 doubles, unpadded) and read_state.cpp:34-67 / write_state.cpp:27-62 (23
