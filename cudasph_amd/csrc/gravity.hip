@@ -524,9 +524,10 @@ __device__ __forceinline__ void far_field(const float* __restrict__ M, float yx,
 // order.
 // The far-field instantiation holds the 48 moment words of an expansion in
 // VGPRs next to the pairwise state: left alone the compiler takes 127
-// registers (4 wavefronts per SIMD); capped at 5 wavefronts it fits 93
-// registers without spilling.  The pairwise instantiation is left unbounded
-// (63 registers, 8 wavefronts).
+// registers (4 wavefronts per SIMD); capped at 5 wavefronts it fits 96
+// registers without spilling (at 6 it spills 15 registers and runs 4-8 %
+// slower).  The pairwise instantiation is left unbounded (63 registers, 8
+// wavefronts).
 template <int JT, bool FAR>
 __global__ __launch_bounds__(kGravBlock) __attribute__((amdgpu_waves_per_eu(FAR ? 5 : 1)))
 void k_gravity_mixed_plummer(const F4* __restrict__ hi,
