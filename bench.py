@@ -358,6 +358,16 @@ def main():
     far = None if args.no_far else run_form(cs, "mixed", n, args.seed, W, K, i_order=args.i_order,
                                             scalars={"grav_far": 1})
 
+    # achievable peaks on this GPU beside the published ones: a 1 GiB copy
+    # kernel and an FMA loop (both the second of two launches)
+    copy_ms = cs.lib().sph_copy_calibration(1 << 30)
+    peaks = {"hbm_copy_GBps": 2 * (1 << 30) / copy_ms / 1e6,
+             "fp32_fma_TFLOPs": cs.lib().sph_fma_peak_form(1),
+             "fp32_fma_scalar_coeff_TFLOPs": cs.lib().sph_fma_peak_form(0),
+             "what": "k_copy16 over 1 GiB (read + written bytes / time); k_fma_peak, 8 wavefronts per SIMD "
+                     "on every CU, 2 flops per single-lane FMA: 16 chains of the accumulate form v_fmac_f32 "
+                     "with vector operands (fp32_fma_TFLOPs) and 8 chains of v_fma_f32 with scalar "
+                     "coefficients"}
     value = n * K / mixed["wall_s"]
     g_ms, g_calls = mixed["timers"].get("gravity", (float("nan"), 0))
     g_avg = g_ms / max(g_calls, 1) / 1e3
@@ -406,6 +416,7 @@ def main():
         "kernel_ms_per_step": {k: v[0] / max(K, 1) for k, v in sorted(mixed["timers"].items())},
         "kernel_ms_per_step_in_order": {k: v[0] / sp for (t, sp) in [mixed["timers_in_order"]]
                                         for k, v in sorted(t.items())},
+        "measured_peaks": peaks,
         "sph_hbm": {"mean_list_length": mixed["kbar"],
                     "measured": "2 steps after the timed region with gravity in order, so each pass runs alone",
                     "passes": sph_bandwidth(mixed["timers_in_order"][0], mixed["timers_in_order"][1], n,
@@ -415,6 +426,7 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profile("k_gravity"),
                      "peak_unpacked": PEAK_FP32_UNPACKED_TFLOPS,
                      "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
+                     "frac_of_measured_fma": achieved / peaks["fp32_fma_TFLOPs"],
                      "per_launch": f"{FLOP_PER_PAIR} flop x {nd}^2 pairs",
                      "launch_ms": g_avg * 1e3},
         "far_field": None if far is None else {

@@ -98,6 +98,8 @@ def lib() -> C.CDLL:
     L.sph_abi_set_params.argtypes = [C.POINTER(Params), i]
     L.sph_copy_calibration.argtypes = [l]
     L.sph_copy_calibration.restype = d
+    L.sph_fma_peak_form.argtypes = [i]
+    L.sph_fma_peak_form.restype = d
     L.sph_adjoint_forward.argtypes = [vp, i, i, C.POINTER(C.c_double)]
     L.sph_adjoint_forward.restype = i
     L.sph_adjoint_backward.argtypes = [vp, vp, vp, vp]

@@ -244,6 +244,33 @@ double sph_copy_calibration(long bytes) {
   }, -1.0);
 }
 
+double sph_fma_peak_form(int form) {
+  return guard([&] {
+    int dev = 0, cus = 0;
+    CS_HIP(hipGetDevice(&dev));
+    CS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    // 8 wavefronts per SIMD on every CU: 4 SIMDs x 8 wavefronts / 4 per block
+    const int blocks = cus * 8, iters = 16384;
+    float* out = nullptr;
+    CS_HIP(hipMalloc(&out, (size_t)blocks * kBlock * sizeof(float)));
+    hipEvent_t e0, e1;
+    CS_HIP(hipEventCreate(&e0));
+    CS_HIP(hipEventCreate(&e1));
+    launch_fma_peak(out, blocks, iters, form, nullptr);
+    CS_HIP(hipEventRecord(e0, nullptr));
+    launch_fma_peak(out, blocks, iters, form, nullptr);
+    CS_HIP(hipEventRecord(e1, nullptr));
+    CS_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    CS_HIP(hipEventDestroy(e0));
+    CS_HIP(hipEventDestroy(e1));
+    CS_HIP(hipFree(out));
+    const double flops = 2.0 * 8.0 * iters * (double)blocks * kBlock;
+    return flops / (ms * 1e-3) / 1e12;
+  }, -1.0);
+}
+
 int sph_state_file_count(const char* path) {
   return guard([&] {
     FILE* f = std::fopen(path, "rb");

@@ -497,6 +497,35 @@ void launch_copy16(const F4* src, F4* dst, long n, hipStream_t s) {
   CS_CHECK_LAUNCH();
 }
 
+// FP32 FMA issue-rate probe: 8 independent chains per lane with runtime
+// coefficients (nothing folds), single-lane v_fma_f32 (this build emits no
+// packed FP32), 2 flops per FMA; each lane writes its sum
+template <int FORM>
+__global__ __launch_bounds__(kBlock) void k_fma_peak(float* __restrict__ out, int iters, float b, float c) {
+  constexpr int kChains = FORM ? 16 : 8;
+  float x[kChains];
+#pragma unroll
+  for (int k = 0; k < kChains; ++k) x[k] = (float)(threadIdx.x + k) * 1e-3f;
+  // per-lane coefficients, so the accumulate form takes vector operands
+  const float y = b + (float)(threadIdx.x & 7) * 1e-6f, z = c + (float)(threadIdx.x & 3) * 1e-9f;
+#pragma unroll 4
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int k = 0; k < kChains; ++k) x[k] = FORM ? __builtin_fmaf(y, z, x[k]) : __builtin_fmaf(x[k], b, c);
+  }
+  float sum = 0.f;
+#pragma unroll
+  for (int k = 0; k < kChains; ++k) sum += x[k];
+  out[(long)blockIdx.x * kBlock + threadIdx.x] = sum;
+}
+void launch_fma_peak(float* out, int blocks, int iters, int form, hipStream_t s) {
+  if (form)
+    hipLaunchKernelGGL(k_fma_peak<1>, dim3(blocks), dim3(kBlock), 0, s, out, iters / 2, 0.999f, 1e-3f);
+  else
+    hipLaunchKernelGGL(k_fma_peak<0>, dim3(blocks), dim3(kBlock), 0, s, out, iters, 0.999f, 1e-3f);
+  CS_CHECK_LAUNCH();
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_set_range(V4<T>* __restrict__ pos, const double* __restrict__ r, int n) {
   const int i = blockIdx.x * kBlock + threadIdx.x;

@@ -151,5 +151,10 @@ void launch_set_range(V4<T>* pos, const double* range, int n, hipStream_t s);
 template <class T>
 void launch_get_range(const V4<T>* pos, double* range, int n, hipStream_t s);
 void launch_copy16(const F4* src, F4* dst, long n, hipStream_t s);
+// FP32 FMA probe: blocks x kBlock lanes, 8 * iters FMAs per lane (form 0:
+// 8 chains of v_fma_f32 with scalar coefficients; form 1: 16 chains of the
+// accumulate form v_fmac_f32 with vector operands); out holds blocks * kBlock
+// floats
+void launch_fma_peak(float* out, int blocks, int iters, int form, hipStream_t s);
 
 }  // namespace cudasph

@@ -244,6 +244,12 @@ int    sph_engine_output(void* e, const char* dir);
  * reference point for scaling the FETCH_SIZE / WRITE_SIZE counters of
  * rocprofv3.  No counterpart in the reference. */
 double sph_copy_calibration(long bytes);
+/* Measured FP32 FMA rate in TFLOP/s (2 flops per FMA) on 8 wavefronts per
+ * SIMD of every CU, single-lane (unpacked) FMAs, the second of two launches
+ * timed: form 0, 8 chains of v_fma_f32 with scalar coefficients; form 1, 16
+ * chains of the accumulate form v_fmac_f32 with vector operands.  The
+ * achievable peak beside the published one.  No counterpart in the reference. */
+double sph_fma_peak_form(int form);
 
 #ifdef __cplusplus
 }
