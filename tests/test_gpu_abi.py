@@ -211,3 +211,17 @@ def test_radix_sort_is_stable(cs):
             d.free()
         order = np.argsort(keys, kind="stable")
         assert np.array_equal(got[:, 0], keys[order]) and np.array_equal(got[:, 1], order), (n, bits)
+
+
+def test_measurement_probes(cs):
+    """The achievable-peak probes the bench reports: a 1 GiB copy and the FMA
+    loop in both forms return finite, plausible rates (a failed launch returns
+    a negative number)."""
+    L = cs.lib()
+    ms = L.sph_copy_calibration(1 << 30)
+    assert 0 < ms < 1e3
+    gbps = 2 * (1 << 30) / ms / 1e6
+    assert 500 < gbps < 20000, gbps
+    for form in (0, 1):
+        t = L.sph_fma_peak_form(form)
+        assert 5 < t < 500, (form, t)
