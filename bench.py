@@ -364,10 +364,13 @@ def main():
     peaks = {"hbm_copy_GBps": 2 * (1 << 30) / copy_ms / 1e6,
              "fp32_fma_TFLOPs": cs.lib().sph_fma_peak_form(1),
              "fp32_fma_scalar_coeff_TFLOPs": cs.lib().sph_fma_peak_form(0),
+             "gravity_pair_body_TFLOPs": cs.lib().sph_pair_body_probe(0) * FLOP_PER_PAIR / 1e12,
              "what": "k_copy16 over 1 GiB (read + written bytes / time); k_fma_peak, 8 wavefronts per SIMD "
                      "on every CU, 2 flops per single-lane FMA: 16 chains of the accumulate form v_fmac_f32 "
                      "with vector operands (fp32_fma_TFLOPs) and 8 chains of v_fma_f32 with scalar "
-                     "coefficients"}
+                     "coefficients; k_pair_body: the gravity kernel's expanded equal-mass pair body alone "
+                     "(11 VALU + rsq) on register-held records, 8 wavefronts per SIMD, in the 20-flop "
+                     "convention: the ceiling of that pair body"}
     value = n * K / mixed["wall_s"]
     g_ms, g_calls = mixed["timers"].get("gravity", (float("nan"), 0))
     g_avg = g_ms / max(g_calls, 1) / 1e3
@@ -427,6 +430,7 @@ def main():
                      "peak_unpacked": PEAK_FP32_UNPACKED_TFLOPS,
                      "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
                      "frac_of_measured_fma": achieved / peaks["fp32_fma_TFLOPs"],
+                     "frac_of_pair_body": achieved / peaks["gravity_pair_body_TFLOPs"],
                      "per_launch": f"{FLOP_PER_PAIR} flop x {nd}^2 pairs",
                      "launch_ms": g_avg * 1e3},
         "far_field": None if far is None else {

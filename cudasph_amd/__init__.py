@@ -100,6 +100,8 @@ def lib() -> C.CDLL:
     L.sph_copy_calibration.restype = d
     L.sph_fma_peak_form.argtypes = [i]
     L.sph_fma_peak_form.restype = d
+    L.sph_pair_body_probe.argtypes = [i]
+    L.sph_pair_body_probe.restype = d
     L.sph_adjoint_forward.argtypes = [vp, i, i, C.POINTER(C.c_double)]
     L.sph_adjoint_forward.restype = i
     L.sph_adjoint_backward.argtypes = [vp, vp, vp, vp]

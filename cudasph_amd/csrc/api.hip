@@ -271,6 +271,31 @@ double sph_fma_peak_form(int form) {
   }, -1.0);
 }
 
+double sph_pair_body_probe(int variant) {
+  return guard([&] {
+    int dev = 0, cus = 0;
+    CS_HIP(hipGetDevice(&dev));
+    CS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int blocks = cus * 8, rounds = 2048;
+    float* out = nullptr;
+    CS_HIP(hipMalloc(&out, (size_t)blocks * kBlock * sizeof(float)));
+    hipEvent_t e0, e1;
+    CS_HIP(hipEventCreate(&e0));
+    CS_HIP(hipEventCreate(&e1));
+    launch_pair_body(out, blocks, rounds, variant, nullptr);
+    CS_HIP(hipEventRecord(e0, nullptr));
+    launch_pair_body(out, blocks, rounds, variant, nullptr);
+    CS_HIP(hipEventRecord(e1, nullptr));
+    CS_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    CS_HIP(hipEventDestroy(e0));
+    CS_HIP(hipEventDestroy(e1));
+    CS_HIP(hipFree(out));
+    return 16.0 * rounds * (double)blocks * kBlock / (ms * 1e-3);
+  }, -1.0);
+}
+
 int sph_state_file_count(const char* path) {
   return guard([&] {
     FILE* f = std::fopen(path, "rb");

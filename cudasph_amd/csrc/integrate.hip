@@ -518,6 +518,52 @@ __global__ __launch_bounds__(kBlock) void k_fma_peak(float* __restrict__ out, in
   for (int k = 0; k < kChains; ++k) sum += x[k];
   out[(long)blockIdx.x * kBlock + threadIdx.x] = sum;
 }
+// The gravity kernel's expanded equal-mass pair body alone (11 VALU + rsq
+// per pair) on 8 records held in registers, 16 pairs per round: no LDS, no
+// memory in the loop, 8 wavefronts per SIMD.
+// The records are wave-uniform, as in the kernel; d_i moves a little each
+// round so nothing is loop-invariant.  VARIANT 1 replaces the rsq by a
+// multiply (12 VALU, no transcendental).  Each lane writes its sums.
+template <int VARIANT>
+__global__ __launch_bounds__(kBlock) void k_pair_body(float* __restrict__ out, int rounds, float4 r0, float4 r1,
+                                                      float4 r2, float4 r3) {
+  float4 rec[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const float4 b = (k & 3) == 0 ? r0 : (k & 3) == 1 ? r1 : (k & 3) == 2 ? r2 : r3;
+    rec[k] = make_float4(b.x + 0.01f * k, b.y - 0.02f * k, b.z + 0.03f * k, b.w + 0.001f * k);
+  }
+  float dx = 1.f + 1e-3f * (threadIdx.x & 63), dy = 0.5f, dz = -0.25f;
+  float sa = 0.f, mx = 0.f, my = 0.f, mz = 0.f, pot = 0.f;
+  for (int it = 0; it < rounds; ++it) {
+    const float ci = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, 4e-6f)));
+    const float bx = -2.f * dx, by = -2.f * dy, bz = -2.f * dz;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const float4 pj = rec[k & 7];
+      const float d2 = __builtin_fmaf(pj.x, bx, __builtin_fmaf(pj.y, by, __builtin_fmaf(pj.z, bz, ci + pj.w)));
+      const float inv = VARIANT == 0 ? __builtin_amdgcn_rsqf(d2) : d2 * 0.5f;
+      const float s3 = inv * (inv * inv);
+      sa += s3;
+      mx = __builtin_fmaf(pj.x, s3, mx);
+      my = __builtin_fmaf(pj.y, s3, my);
+      mz = __builtin_fmaf(pj.z, s3, mz);
+      pot -= inv;
+    }
+    dx += 1e-7f;
+  }
+  out[(long)blockIdx.x * kBlock + threadIdx.x] = sa + mx + my + mz + pot;
+}
+void launch_pair_body(float* out, int blocks, int rounds, int variant, hipStream_t s) {
+  const float4 r0 = make_float4(0.1f, 0.2f, 0.3f, 0.14f), r1 = make_float4(-0.2f, 0.1f, 0.05f, 0.0525f),
+               r2 = make_float4(0.3f, -0.1f, 0.2f, 0.14f), r3 = make_float4(0.05f, 0.05f, -0.3f, 0.095f);
+  if (variant)
+    hipLaunchKernelGGL(k_pair_body<1>, dim3(blocks), dim3(kBlock), 0, s, out, rounds, r0, r1, r2, r3);
+  else
+    hipLaunchKernelGGL(k_pair_body<0>, dim3(blocks), dim3(kBlock), 0, s, out, rounds, r0, r1, r2, r3);
+  CS_CHECK_LAUNCH();
+}
+
 void launch_fma_peak(float* out, int blocks, int iters, int form, hipStream_t s) {
   if (form)
     hipLaunchKernelGGL(k_fma_peak<1>, dim3(blocks), dim3(kBlock), 0, s, out, iters / 2, 0.999f, 1e-3f);

@@ -156,5 +156,8 @@ void launch_copy16(const F4* src, F4* dst, long n, hipStream_t s);
 // accumulate form v_fmac_f32 with vector operands); out holds blocks * kBlock
 // floats
 void launch_fma_peak(float* out, int blocks, int iters, int form, hipStream_t s);
+// the expanded gravity pair body alone: blocks x kBlock lanes, 16 * rounds
+// pairs per lane (variant 1: the rsq replaced by a multiply)
+void launch_pair_body(float* out, int blocks, int rounds, int variant, hipStream_t s);
 
 }  // namespace cudasph

@@ -250,6 +250,11 @@ double sph_copy_calibration(long bytes);
  * chains of the accumulate form v_fmac_f32 with vector operands.  The
  * achievable peak beside the published one.  No counterpart in the reference. */
 double sph_fma_peak_form(int form);
+/* Pairs per second of the mixed gravity kernel's expanded equal-mass pair
+ * body alone (11 VALU + rsq per pair) on records held in registers, 8
+ * wavefronts per SIMD of every CU, the second of two launches timed; variant 1
+ * replaces the rsq by a multiply.  No counterpart in the reference. */
+double sph_pair_body_probe(int variant);
 
 #ifdef __cplusplus
 }

@@ -225,3 +225,6 @@ def test_measurement_probes(cs):
     for form in (0, 1):
         t = L.sph_fma_peak_form(form)
         assert 5 < t < 500, (form, t)
+    for variant in (0, 1):
+        pairs = L.sph_pair_body_probe(variant)
+        assert 1e11 < pairs < 1e14, (variant, pairs)
