@@ -48,3 +48,40 @@ def test_sph_force_evaluation_bit_exact_at_full_size(cs, orc, cfg, n):
     scale = (m[:, None] * np.abs(bd[:, :3])).sum(0)
     assert np.all(np.abs(total) <= 1e-5 * scale), (total, scale)
     print(f"  mean list length {cnt[:n].mean():.2f}, ngb_tot {tot}", flush=True)
+
+
+def test_device_tree_same_neighbour_sets_at_full_size(cs):
+    """At N = 2^20 the 24-bit coordinate draws tie many times per axis, so the
+    device tree build (the mixed form's default) may pick other tied
+    particles as nodes than the reference's quicksort build does.  The
+    neighbour sets of every particle must still be the host tree's: lists
+    built from both trees on the same positions and ranges, compared as
+    sorted (particle, neighbour) keys."""
+    n = 1048576
+    print(f"full-size device tree n={n}", flush=True)
+    p = cs.preset_params(3, n)
+    pos, vel, hyd = cs.generate_ic(3, n, 12345)
+    e = cs.Engine("plain", p)
+    e.load_ic(pos, vel, hyd)
+
+    def keys():
+        off, cnt, lst = e.get("ngb_offset")[:n], e.get("ngb_count")[:n], e.get("ngb_list")
+        owner = np.repeat(np.arange(n, dtype=np.int64), cnt)
+        idx = np.repeat(off.astype(np.int64) - np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt) + np.arange(cnt.sum())
+        return np.sort(owner * (1 << 24) + lst[idx].astype(np.int64))
+
+    e.set_scalar("tree_gpu", 0)
+    e.run_pass(cs.TREE)
+    e.run_pass(cs.SOLVE)
+    e.run_pass(cs.OFFSETS)
+    e.run_pass(cs.FIND)
+    host = keys()
+    e.set_scalar("tree_gpu", 1)
+    e.run_pass(cs.TREE)
+    e.run_pass(cs.OFFSETS)
+    e.run_pass(cs.FIND)
+    dev = keys()
+    assert host.size == dev.size, (host.size, dev.size)
+    assert np.array_equal(host, dev), "neighbour sets differ between the two tree builds"
+    print(f"  {host.size} (particle, neighbour) pairs identical", flush=True)
+    e.close()
