@@ -85,3 +85,34 @@ def test_device_tree_same_neighbour_sets_at_full_size(cs):
     assert np.array_equal(host, dev), "neighbour sets differ between the two tree builds"
     print(f"  {host.size} (particle, neighbour) pairs identical", flush=True)
     e.close()
+
+
+@pytest.mark.parametrize("cfg", [3, 5])
+def test_far_field_against_fp64_at_full_size(cs, cfg):
+    """The mixed form's optional far field at the benchmark size, against FP64
+    gravity on the same positions (the FP64 form on the GPU): the maximum
+    per-particle acceleration error stays within the pairwise FP32 kernel's
+    tolerance and the median within 1e-7."""
+    n = 1048576
+    print(f"full-size far field cfg={cfg} n={n}", flush=True)
+    e = cs.make_engine("mixed", cfg, n, 12345, ph_sort=True, i_order=0)
+    e.set_scalar("grav_far", 1)
+    e.run_pass(cs.GRAVITY)
+    g = e.get("grav")[:n].astype(np.float64)
+    pos64, vel64 = e.get("pos64")[:n, :3].copy(), e.get("vel64")[:n].copy()
+    p = e.params.copy()
+    e.close()
+    f = cs.Engine("fp64", p)
+    P = f.get("pos")
+    P[:n, :3] = pos64
+    f.set("pos", P)
+    V = f.get("vel")
+    V[:n] = vel64
+    f.set("vel", V)
+    f.run_pass(cs.GRAVITY)
+    go = f.get("grav")[:n]
+    f.close()
+    rms = np.sqrt((go[:, :3] ** 2).sum(1).mean())
+    err = np.linalg.norm(g[:, :3] - go[:, :3], axis=1) / np.maximum(np.linalg.norm(go[:, :3], axis=1), 1e-3 * rms)
+    print(f"  median {np.median(err):.2e} max {err.max():.2e}", flush=True)
+    assert err.max() < 1e-5 and np.median(err) < 1e-7, (err.max(), np.median(err))
