@@ -116,3 +116,27 @@ def test_far_field_against_fp64_at_full_size(cs, cfg):
     err = np.linalg.norm(g[:, :3] - go[:, :3], axis=1) / np.maximum(np.linalg.norm(go[:, :3], axis=1), 1e-3 * rms)
     print(f"  median {np.median(err):.2e} max {err.max():.2e}", flush=True)
     assert err.max() < 1e-5 and np.median(err) < 1e-7, (err.max(), np.median(err))
+
+
+def test_energy_drift_against_fp64_at_full_size(cs):
+    """The benchmark's accuracy condition at its own size: over init + 2 steps
+    of the polytrope pair at N = 2^20, the mixed and plain forms keep
+    |E_form(k) - E_fp64(k)| / |E_fp64(0)| / k below 1e-6."""
+    n = 1048576
+    print(f"full-size energy drift n={n}", flush=True)
+    energies = {}
+    for form in ("fp64", "mixed", "plain"):
+        e = cs.make_engine(form, 3, n, 12345, ph_sort=True, i_order=0)
+        e.init()
+        es = [e.stats()["E"]]
+        for _ in range(2):
+            e.step()
+            es.append(e.stats()["E"])
+        energies[form] = np.array(es)
+        e.close()
+        print(f"  {form} done", flush=True)
+    ref = energies["fp64"]
+    for form in ("mixed", "plain"):
+        drift = np.abs(energies[form][1:] - ref[1:]) / abs(ref[0]) / np.arange(1, len(ref))
+        print(f"  {form} drift per step {drift.max():.2e}", flush=True)
+        assert drift.max() < 1e-6, (form, drift)
