@@ -622,13 +622,18 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
     } else {
       __syncthreads();
     }
-    // stage: this wavefront's sub-tiles, each relative to its middle
-    // record, and each sub-tile's squared radius about that record
+    // stage: this wavefront's sub-tiles, each relative to its middle real
+    // record, and each sub-tile's squared radius about that record.  A
+    // padding record (at 1e30) as origin would collapse every real offset
+    // to -1e30 and make its squared radius infinite; the tile test would
+    // then pass for every wavefront and the real records would add nothing.
+    // A sub-tile of padding only keeps its middle record.
     for (int q = threadIdx.x; q < JT; q += kGravBlock) {
       const int sub = q / kTile, sub0 = sub * kTile;
       float r2 = 0.f;
       if (sub0 < cnt) {
-        const F4 c = hi[t0 + sub0 + min(kTile, cnt - sub0) / 2];
+        const int nj = min(kTile, cnt - sub0), nr = max(0, min(nj, n_real - (t0 + sub0)));
+        const F4 c = hi[t0 + sub0 + (nr > 0 ? nr : nj) / 2];
         if (q < cnt) {
           const F4 h = hi[t0 + q], l = lo[t0 + q];
           F4 d{(h.x - c.x) + l.x, (h.y - c.y) + l.y, (h.z - c.z) + l.z, h.w};

@@ -216,14 +216,20 @@ def test_mixed_form_small_sizes(cs, orc, n):
     assert np.isfinite(e.get("bdots")[:n]).all()
 
 
+@pytest.mark.parametrize("n", [16384, 5000, 16404])
+@pytest.mark.parametrize("far", [0, 1])
 @pytest.mark.parametrize("equal", [True, False])
-def test_mixed_gravity_against_fp64(cs, orc, equal):
+def test_mixed_gravity_against_fp64(cs, orc, equal, far, n):
     """The mixed gravity pass against FP64 gravity on the same positions, with
     equal masses (the unit-mass pair body) and with masses spread over a
     factor of 3 (the general body); both Peano-Hilbert ordered, so both the
-    tile-relative and the split forms are taken."""
-    n = 16384
+    tile-relative and the split forms are taken.  N = 5000 and 16404 leave 8
+    and 20 real records in the last 64-record sub-tile, whose middle record
+    is padding: the sub-tile's origin must be a real record.  Pairwise and
+    with the multipole far field; the total force must vanish to rounding."""
+    print(f"mixed gravity vs fp64 n={n} equal={equal} far={far}", flush=True)
     e = cs.make_engine("mixed", 2, n, 23, ph_sort=True)
+    e.set_scalar("grav_far", far)
     v64 = e.get("vel64")
     if not equal:
         rng = np.random.default_rng(5)
@@ -248,6 +254,9 @@ def test_mixed_gravity_against_fp64(cs, orc, equal):
     err = np.linalg.norm(g[:, :3] - go[:, :3], axis=1) / np.maximum(np.linalg.norm(go[:, :3], axis=1), 1e-3 * rms)
     assert err.max() < 1e-5, err.max()
     np.testing.assert_allclose(g[:, 3], go[:, 3], rtol=2e-6)
+    m = v64[:n, 3:4]
+    ftot = np.abs((m * g[:, :3]).sum(0)).max()
+    assert ftot <= 1e-5 * (m[:, 0] * np.linalg.norm(go[:, :3], axis=1)).sum(), ftot
 
 
 @pytest.mark.parametrize("cfg,n,equal", [(3, 16384, True), (3, 16384, False), (3, 16000, True),
@@ -376,3 +385,31 @@ def test_state_file_round_trip(cs, tmp_path):
     assert np.array_equal(h1[:, [0, 2]], h2[:, [0, 2]])
     np.testing.assert_allclose(h2[:, 3], h1[:, 3], rtol=1e-6)   # mu goes through CGS amu and back
     assert e2.scalar("dt") == e.scalar("dt") and e2.scalar("iteration") == e.scalar("iteration")
+
+
+def test_mixed_state_file_keeps_fp64_state(cs, tmp_path):
+    """The mixed form integrates x, v and u in FP64: its state file carries
+    those doubles (not their FP32 views), so a read gives back pos64, vel64
+    and u64 bit for bit after two steps have moved them off FP32 values."""
+    n = 5000
+    e = cs.make_engine("mixed", 3, n, 3, ph_sort=True)
+    e.init()
+    e.step(2)
+    p64, v64, u64 = e.get("pos64")[:n].copy(), e.get("vel64")[:n].copy(), e.get("u64")[:n].copy()
+    assert (p64[:, :3] != p64[:, :3].astype(np.float32)).any(), "positions still FP32-representable"
+    assert (u64 != u64.astype(np.float32)).any(), "u still FP32-representable"
+    f = tmp_path / "restart.bin"
+    e.write_state(f)
+    rec = np.fromfile(f, dtype=np.float64, offset=124).reshape(n, 23)
+    assert np.array_equal(rec[:, 0:3], p64[:, :3])
+    assert np.array_equal(rec[:, 6:9], v64[:, :3])
+    assert np.array_equal(rec[:, 3], v64[:, 3])
+    assert np.array_equal(rec[:, 12], u64)
+    e2 = cs.Engine("mixed", e.params)
+    e2.read_state(f)
+    assert np.array_equal(e2.get("pos64")[:n, :3], p64[:, :3])
+    assert np.array_equal(e2.get("vel64")[:n], v64)
+    assert np.array_equal(e2.get("u64")[:n], u64)
+    assert np.array_equal(e2.get("pos")[:n, :3], e.get("pos")[:n, :3])
+    e.close()
+    e2.close()
