@@ -125,13 +125,33 @@ void write_state_from(EngineBase* e, FILE* f, int n, int nd) {
   e->get("bdots", bd.data(), (long)(nd * sizeof(V4<T>)));
   e->get("grav", gr.data(), (long)(nd * sizeof(V4<T>)));
   e->get("dots", dd.data(), (long)(nd * sizeof(V4<T>)));
+  // the mixed form integrates positions, velocities and u in FP64: the file
+  // (23 doubles per body) keeps that state, so a restart loses nothing
+  const bool mixed = e->get_scalar("form") == SPH_FORM_MIXED;
+  std::vector<D4> p64(mixed ? nd : 0), v64(mixed ? nd : 0);
+  std::vector<double> u64(mixed ? nd : 0);
+  if (mixed) {
+    e->get("pos64", p64.data(), (long)(nd * sizeof(D4)));
+    e->get("vel64", v64.data(), (long)(nd * sizeof(D4)));
+    e->get("u64", u64.data(), (long)(nd * sizeof(double)));
+  }
   for (int i = 0; i < n; ++i) {
-    const double rec[23] = {(double)pos[i].x, (double)pos[i].y, (double)pos[i].z, (double)vel[i].w,
-                            (double)pos[i].w * 0.5, (double)hyd[i].x, (double)vel[i].x, (double)vel[i].y,
-                            (double)vel[i].z, (double)bd[i].x, (double)bd[i].y, (double)bd[i].z,
-                            (double)hyd[i].z, (double)bd[i].w, (double)gr[i].x, (double)gr[i].y,
-                            (double)gr[i].z, (double)gr[i].w, (double)hyd[i].w * kAmuCgs, -1.0, -1.0, -1.0,
-                            (double)dd[i].x};
+    double rec[23] = {(double)pos[i].x, (double)pos[i].y, (double)pos[i].z, (double)vel[i].w,
+                      (double)pos[i].w * 0.5, (double)hyd[i].x, (double)vel[i].x, (double)vel[i].y,
+                      (double)vel[i].z, (double)bd[i].x, (double)bd[i].y, (double)bd[i].z,
+                      (double)hyd[i].z, (double)bd[i].w, (double)gr[i].x, (double)gr[i].y,
+                      (double)gr[i].z, (double)gr[i].w, (double)hyd[i].w * kAmuCgs, -1.0, -1.0, -1.0,
+                      (double)dd[i].x};
+    if (mixed) {
+      rec[0] = p64[i].x;
+      rec[1] = p64[i].y;
+      rec[2] = p64[i].z;
+      rec[3] = v64[i].w;
+      rec[6] = v64[i].x;
+      rec[7] = v64[i].y;
+      rec[8] = v64[i].z;
+      rec[12] = u64[i];
+    }
     std::fwrite(rec, 8, 23, f);
   }
 }
