@@ -58,6 +58,11 @@ def lib() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: run `python -m cudasph_amd.build` first")
+    from . import build as _build
+    if _build.is_stale():
+        import sys
+        sys.stderr.write(f"warning: {LIB_PATH} was built from other sources than csrc/ holds now; "
+                         "run `python -m cudasph_amd.build`\n")
     L = C.CDLL(str(LIB_PATH))
     vp, cp, i, l, d = C.c_void_p, C.c_char_p, C.c_int, C.c_long, C.c_double
     L.sph_params_default.argtypes = [C.POINTER(Params), i]

@@ -7,6 +7,7 @@ that want FMAs spell them out) and with packed FP32 instructions disabled.
 """
 from __future__ import annotations
 
+import hashlib
 import os
 import subprocess
 import sys
@@ -39,9 +40,48 @@ def _run(cmd: list[str]) -> None:
         sys.stderr.write("\n".join(noise) + "\n")
 
 
+MARKER = b"CUDASPH_SOURCE_HASH:"
+
+
+def source_hash() -> str:
+    """sha256 over the library's sources (csrc/ and include/), in name order."""
+    h = hashlib.sha256()
+    files = sorted(list(SRC.iterdir()) + list((PKG.parent / "include").glob("*.h")))
+    for f in files:
+        if f.suffix in (".hip", ".cpp", ".h"):
+            h.update(f.name.encode() + b"\0" + f.read_bytes() + b"\0")
+    return h.hexdigest()[:32]
+
+
+def built_hash() -> str | None:
+    """The source hash recorded in the built library, or None."""
+    if not LIB.exists():
+        return None
+    data = LIB.read_bytes()
+    k = data.find(MARKER)
+    if k < 0:
+        return None
+    return data[k + len(MARKER): k + len(MARKER) + 32].decode("ascii", "replace")
+
+
+def is_stale() -> bool:
+    """True when the library is missing or was built from other sources."""
+    return built_hash() != source_hash()
+
+
+def ensure_built(verbose: bool = True) -> Path:
+    """Build the library when it is missing or older than its sources."""
+    if is_stale():
+        return build(verbose)
+    return LIB
+
+
 def build(verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
-    jobs = []
+    stamp = OBJ / "source_hash.c"
+    stamp.write_text('const char sph_source_hash_marker[] __attribute__((used)) = "'
+                     + MARKER.decode() + source_hash() + '";\n')
+    jobs = [["gcc", "-O2", "-fPIC", "-c", str(stamp), "-o", str(OBJ / "source_hash.c.o")]]
     for src in sorted(SRC.glob("*.hip")):
         jobs.append([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(OBJ / (src.stem + ".o"))])
     for src in sorted(SRC.glob("*.cpp")):

@@ -1,5 +1,7 @@
 """Test configuration: puts the repo root on sys.path, registers the `gpu`
-marker and builds the libraries if they are missing."""
+marker and builds the libraries when they are missing or older than their
+sources (libcudasph.so records a hash of its sources; the oracle's Makefile
+compares times)."""
 import sys
 from pathlib import Path
 
@@ -17,15 +19,13 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def orc():
     import oracle
-    if not oracle.LIB_PATH.exists():
-        oracle.build()
+    oracle.build()
     return oracle
 
 
 @pytest.fixture(scope="session")
 def cs():
+    from cudasph_amd import build
+    build.ensure_built()
     import cudasph_amd
-    if not cudasph_amd.LIB_PATH.exists():
-        from cudasph_amd import build
-        build.build()
     return cudasph_amd
