@@ -23,6 +23,26 @@ def m4_cumulative(x):
     return np.where(x < 0, 0.0, np.where(x < 1, a, np.where(x < 2, b, 1.0)))
 
 
+def phi_m4(q):
+    """Softened potential kernel of the M4 spline (Price & Monaghan 2007,
+    eq. A2-A3): the potential of unit mass at r is phi(r/h)/h, -1/q outside."""
+    q = np.asarray(q, float)
+    a = 2 / 3 * q**2 - 0.3 * q**4 + 0.1 * q**5 - 1.4
+    qs = np.where(q > 0, q, 1.0)
+    b = 4 / 3 * q**2 - q**3 + 0.3 * q**4 - q**5 / 30 - 1.6 + 1 / (15 * qs)
+    return np.where(q < 1, a, np.where(q < 2, b, -1 / qs))
+
+
+def dphi_m4(q):
+    """d phi / dq of phi_m4 (the force kernel: q^2 times it is the mass
+    fraction of the spline inside q)."""
+    q = np.asarray(q, float)
+    a = 4 / 3 * q - 1.2 * q**3 + 0.5 * q**4
+    qs = np.where(q > 0, q, 1.0)
+    b = 8 / 3 * q - 3 * q**2 + 1.2 * q**3 - q**4 / 6 - 1 / (15 * qs**2)
+    return np.where(q < 1, a, np.where(q < 2, b, 1 / qs**2))
+
+
 def window(u):
     """smoothed-count window ig(u) and its u-derivative"""
     u = np.asarray(u, float)
@@ -67,11 +87,28 @@ def test_density_matches_fp64_brute_force(cs, orc, cfg):
     divv = (mdw * (dr * dv).sum(-1)).sum(1) / rho / h**5
     curl = np.cross(dr, dv)
     curlv = np.linalg.norm((mdw[..., None] * curl).sum(1), axis=1) / rho / h**5
+    # Psi: h-derivative of the M4 softened potential, phi(q)/h with the
+    # textbook phi (Price & Monaghan 2007, eq. A2-A3), normalised like Omega:
+    # Psi = sum_j m_j [-(phi + q phi')](u) / h / (-sum digdh)
+    dphidh = -(phi_m4(np.where(inside, u, 2.5)) + u * dphi_m4(np.where(inside, u, 2.5))) * inside
+    psi = (m[None, :] * dphidh).sum(1) / h / den
     hyd, dots = o.get("hydro")[:n], o.get("dots")[:n]
     np.testing.assert_allclose(hyd[:, 0], rho, rtol=1e-10)
     np.testing.assert_allclose(dots[:, 2], omega, rtol=1e-9)
+    np.testing.assert_allclose(dots[:, 3], psi, rtol=1e-9)
     np.testing.assert_allclose(dots[:, 0], divv, rtol=1e-8, atol=1e-10 * np.abs(divv).max())
     np.testing.assert_allclose(dots[:, 1], curlv, rtol=1e-8, atol=1e-10 * np.abs(curlv).max() + 1e-300)
+
+
+def test_softened_potential_kernel(cs, orc):
+    """phi_m4 above is the potential of the M4 density: by Gauss, q^2 phi'(q)
+    is the spline's mass inside q; phi is continuous and Keplerian past 2."""
+    q = np.linspace(1e-3, 2.5, 2001)
+    np.testing.assert_allclose(q**2 * dphi_m4(q), m4_cumulative(q), rtol=1e-12, atol=1e-15)
+    for x in (1.0, 2.0):
+        assert phi_m4(x - 1e-12) == pytest.approx(phi_m4(x + 1e-12), abs=1e-10)
+    h = 1e-6
+    np.testing.assert_allclose((phi_m4(q + h) - phi_m4(q - h)) / (2 * h), dphi_m4(q), rtol=1e-6, atol=1e-8)
 
 
 def test_density_fp32_tracks_fp64(cs, orc):
@@ -179,6 +216,110 @@ def test_hydro_with_reaction_conserves_momentum(cs, orc, cfg, bits):
     assert np.all(np.abs((m[:, None] * a).sum(0)) <= tol * gross)
     torque = (m[:, None] * np.cross(x, a)).sum(0)
     assert np.all(np.abs(torque) <= tol * (m * np.linalg.norm(x, axis=1) * np.linalg.norm(a, axis=1)).sum())
+
+
+def hydro_state(cs, orc, cfg, n, alpha=None, beta=None, seed=17):
+    """FP64 oracle after the passes of one SPH force evaluation (tree,
+    h-solve, lists, density, EOS, hydro, reaction), phantom node removed."""
+    p = cs.preset_params(cfg, n)
+    p.fix_phantom = 1
+    if alpha is not None:
+        p.av_alpha, p.av_beta = alpha, beta
+    pos, vel, hyd = cs.generate_ic(cfg, n, seed)
+    o = orc.Oracle(64, p)
+    o.load_ic(pos, vel, hyd)
+    for pid in (orc.TREE, orc.SOLVE, orc.OFFSETS, orc.FIND, orc.DENSITY, orc.EOS, orc.HYDRO):
+        o.run_pass(pid)
+    udot_hydro = o.get("bdots")[:n, 3].copy()
+    o.run_pass(orc.REACTION)
+    return o, udot_hydro
+
+
+def rates_along_velocity(pos, vel, n, delta):
+    """Central differences in tau of rho_i(tau) = sum_j m_j W(|r_ij + tau v_ij| / h_i) / h_i^3
+    and N_i(tau) = sum_j ig(|r_ij + tau v_ij| / h_i), h_i held: the rates of
+    change of density and smoothed neighbour count along the velocity field.
+    Textbook M4 and window (above), FP64 brute force over all pairs."""
+    x, v, m = pos[:n, :3], vel[:n, :3], vel[:n, 3]
+    h = 0.5 * pos[:n, 3]
+
+    def sums(tau):
+        r = np.sqrt(((x[:, None] - x[None, :] + tau * (v[:, None] - v[None, :])) ** 2).sum(-1))
+        u = r / h[:, None]
+        inside = u < 2
+        w, _ = m4(np.where(inside, u, 2.5))
+        ig, _ = window(np.where(inside, u, 2.5))
+        return (m[None, :] * w * inside).sum(1) / h**3, (ig * inside).sum(1)
+
+    rp, npl = sums(delta)
+    rm, nm = sums(-delta)
+    return (rp - rm) / (2 * delta), (npl - nm) / (2 * delta)
+
+
+@pytest.mark.parametrize("cfg", [2, 5])
+def test_hydro_udot_is_the_adiabatic_rate(cs, orc, cfg):
+    """With the viscosity off (alpha = beta = 0) the hydro pass's du_i/dt is
+    the grad-h form of the first law at fixed h,
+        du_i/dt = (P_i / rho_i^2) [d rho_i/dt - Omega_i dN_i/dt],
+    rho_i and N_i = sum_j ig(u_ij) rates taken along the velocities by central
+    differences of brute-force sums (dev_hydro_dots.cu:76-80).  A wrong power
+    of h or a missing Omega in u-dot fails this."""
+    n = 1024
+    o, udot = hydro_state(cs, orc, cfg, n, alpha=0.0, beta=0.0)
+    pos, vel = o.get("pos"), o.get("vel")
+    hyd, dots = o.get("hydro")[:n], o.get("dots")[:n]
+    vmax = np.abs(vel[:n, :3]).max()
+    drho, dN = rates_along_velocity(pos, vel, n, 2e-6 * pos[:n, 3].min() / vmax)
+    expect = hyd[:, 1] / hyd[:, 0] ** 2 * (drho - dots[:, 2] * dN)
+    scale = np.abs(expect).max()
+    err = np.abs(udot - expect) / np.maximum(np.abs(expect), 1e-3 * scale)
+    assert err.max() < 1e-6, err.max()
+    assert np.abs(udot - expect).max() < 1e-8 * scale
+
+
+@pytest.mark.parametrize("cfg", [2, 5])
+def test_hydro_energy_identity_with_viscosity(cs, orc, cfg):
+    """With viscosity on, the pair terms of the acceleration (own loop and
+    reaction) and of du/dt cancel in the total energy rate except for the
+    Psi (softening) term:
+        sum_i m_i (v_i . a_i + du_i/dt) = 1/2 sum_i m_i Psi_i dN_i/dt.
+    Holds only if Pi enters a and du/dt with the same factor 1/2, if P/rho^2
+    Omega and Psi/2 multiply dig/h^2 as in dev_hydro_dots.cu:64-80, and if
+    the reaction adds m_i acc_ij r_ij to j (sph_accelerations.cpp:104-115)."""
+    n = 1024
+    o, _ = hydro_state(cs, orc, cfg, n)
+    pos, vel = o.get("pos"), o.get("vel")
+    bd, dots = o.get("bdots")[:n], o.get("dots")[:n]
+    m, v = vel[:n, 3], vel[:n, :3]
+    vmax = np.abs(v).max()
+    _, dN = rates_along_velocity(pos, vel, n, 2e-6 * pos[:n, 3].min() / vmax)
+    work = m * (v * bd[:, :3]).sum(1)
+    lhs = (work + m * bd[:, 3]).sum()
+    rhs = 0.5 * (m * dots[:, 3] * dN).sum()
+    gross = np.abs(work).sum() + np.abs(m * bd[:, 3]).sum()
+    assert abs(lhs - rhs) <= 1e-11 * gross, (lhs, rhs, gross)
+    # the identity is not empty: viscosity and the Psi term both act
+    o0, _ = hydro_state(cs, orc, cfg, n, alpha=0.0, beta=0.0)
+    assert np.abs(o0.get("bdots")[:n] - bd).max() > 1e-3 * np.abs(bd).max()
+    assert abs(rhs) > 1e-6 * gross
+
+
+def test_viscosity_pure_compression_switch_is_one(orc):
+    """Balsara switch |div v| / (|div v| + |curl v| + 1e-4 c / h) -> 1 under
+    pure compression (curl 0, |div v| >> c / h): Pi then takes its unswitched
+    value (P_i/rho_i^2 + P_j/rho_j^2) (-alpha mu + beta mu^2), with
+    mu = v_r / c_bar, v_r = (r . v)/|r| and c_bar = (c_i + c_j)/2
+    (dev_art_visc.cu:15-29)."""
+    drdv, ds, alpha, beta = -0.05, 0.1, 1.0, 2.0
+    hi, hj = [1.0, 0.6, 1.0, 0.6], [1.2, 0.9, 1.0, 0.6]
+    ci, cj = np.sqrt(5 / 3 * 0.6 / 1.0), np.sqrt(5 / 3 * 0.9 / 1.2)
+    mu = (drdv / ds) / (0.5 * (ci + cj))
+    unswitched = (0.6 / 1.0**2 + 0.9 / 1.2**2) * (-alpha * mu + beta * mu**2)
+    pi, _ = orc.art_visc(64, alpha, beta, 0.2, 0.25, ds, drdv, hi, [-1e6, 0, 0, 0], hj, [-1e6, 0, 0, 0])
+    assert pi == pytest.approx(unswitched, rel=1e-8)
+    # a switch of 1/2 on both sides (|curl v| = |div v|, c/h negligible) halves mu
+    pi, _ = orc.art_visc(64, alpha, 0.0, 0.2, 0.25, ds, drdv, hi, [-1e6, 1e6, 0, 0], hj, [-1e6, 1e6, 0, 0])
+    assert pi == pytest.approx((0.6 + 0.9 / 1.44) * (-alpha * 0.5 * mu), rel=1e-8)
 
 
 # ---- gravity (sph_grav_kernel.cu) ------------------------------------------
