@@ -225,25 +225,11 @@ void fft3d(std::vector<std::complex<double>>& g, int L, int sign) {
   for (long yz = 0; yz < L2; ++yz) fft1d(&g[yz], L, L2, sign);                             // x
 }
 
-// config 5: uniform sphere R = 1, M = 1; Gaussian velocity field with
-// P(k) ~ k^-4 on a periodic 128^3 grid over [-1,1]^3 (white noise filtered by
-// k^-2 in Fourier space), trilinear to particles, scaled to rms Mach 2 with
-// c_s = sqrt(gamma (gamma-1) u), u = 1
-void ic_turbulent(int n, double ngb, Rng& rng, double* pos, double* vel, double* hyd, int L = 128) {
-  const double number_density = n / (4.0 / 3.0 * kPi);
-  const double range = range_for(ngb, number_density);
-  for (int i = 0; i < n; ++i) {
-    double x, y, z;
-    do {
-      x = 2 * rng.uniform() - 1;
-      y = 2 * rng.uniform() - 1;
-      z = 2 * rng.uniform() - 1;
-    } while (x * x + y * y + z * z > 1.0);
-    put(pos, i, x, y, z, range);
-    put(hyd, i, 0, 0, 1.0, 0.6);
-  }
+// Gaussian random field with P(k) ~ k^-4 on a periodic L^3 grid (white
+// noise filtered by k^-2 in Fourier space), one per velocity component, from
+// the stream after the particle positions
+void turbulent_field(Rng& rng, int L, std::vector<double> (&field)[3]) {
   const long L3 = (long)L * L * L;
-  std::vector<double> field[3];
   for (int c = 0; c < 3; ++c) {
     std::vector<std::complex<double>> g(L3);
     for (long k = 0; k < L3; ++k) g[k] = rng.gauss();
@@ -259,6 +245,32 @@ void ic_turbulent(int n, double ngb, Rng& rng, double* pos, double* vel, double*
     field[c].resize(L3);
     for (long k = 0; k < L3; ++k) field[c][k] = g[k].real();
   }
+}
+
+// uniform positions in the unit sphere by rejection from [-1,1]^3
+void sphere_positions(int n, double range, Rng& rng, double* pos) {
+  for (int i = 0; i < n; ++i) {
+    double x, y, z;
+    do {
+      x = 2 * rng.uniform() - 1;
+      y = 2 * rng.uniform() - 1;
+      z = 2 * rng.uniform() - 1;
+    } while (x * x + y * y + z * z > 1.0);
+    put(pos, i, x, y, z, range);
+  }
+}
+
+constexpr int kTurbGrid = 128;
+
+// config 5: uniform sphere R = 1, M = 1; the k^-4 field on a periodic 128^3
+// grid over [-1,1]^3, trilinear to particles, scaled to rms Mach 2 with
+// c_s = sqrt(gamma (gamma-1) u), u = 1
+void ic_turbulent(int n, double ngb, Rng& rng, double* pos, double* vel, double* hyd, int L = kTurbGrid) {
+  const double number_density = n / (4.0 / 3.0 * kPi);
+  sphere_positions(n, range_for(ngb, number_density), rng, pos);
+  for (int i = 0; i < n; ++i) put(hyd, i, 0, 0, 1.0, 0.6);
+  std::vector<double> field[3];
+  turbulent_field(rng, L, field);
   double sum2 = 0;
   for (int i = 0; i < n; ++i) {
     double v[3];
@@ -285,6 +297,21 @@ void ic_turbulent(int n, double ngb, Rng& rng, double* pos, double* vel, double*
 }
 
 }  // namespace
+
+// the velocity grid of config 5 for n particles and this seed (the same
+// stream as sph_ic_generate): 3 L^3 doubles, component-major, then x-major
+// (index ((x L + y) L + z)), before interpolation and scaling
+extern "C" int sph_ic_turbulent_field(int n, uint64_t seed, double* field3) {
+  if (n <= 0 || !field3) return -1;
+  Rng rng(seed);
+  std::vector<double> pos(4 * (size_t)n);
+  sphere_positions(n, 0.0, rng, pos.data());
+  std::vector<double> field[3];
+  turbulent_field(rng, kTurbGrid, field);
+  const size_t L3 = (size_t)kTurbGrid * kTurbGrid * kTurbGrid;
+  for (int c = 0; c < 3; ++c) std::copy(field[c].begin(), field[c].end(), field3 + c * L3);
+  return 0;
+}
 
 extern "C" int sph_ic_generate(int config, int n, uint64_t seed, double* pos4, double* vel4, double* hydro4) {
   if (n <= 0) return -1;

@@ -156,6 +156,12 @@ int         sph_params_preset(sph_params* p, int config, int n_bodies);
  * every x86-64 host with the same libm. */
 int         sph_ic_generate(int config, int n, uint64_t seed, double* pos4, double* vel4,
                             double* hydro4);
+/* The velocity grid behind config 5's particle velocities, for the same n
+ * and seed: three 128^3 fields (component-major, index (x*128 + y)*128 + z
+ * over [-1,1)^3, periodic) of white noise filtered by k^-2, so P(k) ~ k^-4,
+ * before the trilinear interpolation and the scaling to rms Mach 2.
+ * field3 receives 3*128^3 doubles.  Returns 0, or -1 on bad arguments. */
+int sph_ic_turbulent_field(int n, uint64_t seed, double* field3);
 
 const char* sph_last_error(void);
 
