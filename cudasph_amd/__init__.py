@@ -89,6 +89,8 @@ def lib() -> C.CDLL:
     L.sph_engine_sync.restype = i
     L.sph_engine_stats.argtypes = [vp, C.POINTER(C.c_double)]
     L.sph_engine_stats.restype = i
+    L.sph_engine_statistics.argtypes = [vp, C.POINTER(C.c_double)]
+    L.sph_engine_statistics.restype = i
     L.sph_engine_timers.argtypes = [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_long), i]
     L.sph_engine_timers.restype = i
     L.sph_engine_reset_timers.argtypes = [vp]
@@ -168,6 +170,12 @@ def _err(what: str):
 
 
 STATS_KEYS = ("T", "U", "W", "E", "J", "Fx", "Fy", "Fz", "Tx", "Ty", "Tz")
+# sph_engine_statistics' layout (SPH_STAT_* in include/cudasph.h)
+STATISTICS_KEYS = STATS_KEYS + (
+    "Jx", "Jy", "Jz", "xmin", "ymin", "zmin", "xmax", "ymax", "zmax", "m_tot",
+    "xcom", "ycom", "zcom", "vxcom", "vycom", "vzcom", "Rcm", "Vcm",
+    "ngb_mean", "ngb_sigma", "ngb_min", "ngb_max", "count_mismatch", "max_count",
+    "phantom_reads", "phantom_ngb", "overflow_reruns", "iteration", "time")
 
 
 class Engine:
@@ -270,6 +278,15 @@ class Engine:
         if lib().sph_engine_stats(self._h, out) != 0:
             raise _err("stats")
         return dict(zip(STATS_KEYS, list(out)))
+
+    def statistics(self) -> dict:
+        """What system_statistics.cpp:4-82 reports (energies, box, centre of
+        mass, |J|, neighbour-count mean/sigma/min/max) plus the list counters
+        of the last find_ngb, from the last STATS pass."""
+        out = (C.c_double * len(STATISTICS_KEYS))()
+        if lib().sph_engine_statistics(self._h, out) != len(STATISTICS_KEYS):
+            raise _err("statistics")
+        return dict(zip(STATISTICS_KEYS, list(out)))
 
     def timers(self) -> dict:
         k = 64

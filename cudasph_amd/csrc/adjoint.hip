@@ -585,6 +585,34 @@ void Engine<T>::adj_load(DBuf<V4<T>>* src[4], long slot) {
 }
 
 template <class T>
+void Engine<T>::adj_keep_end() {
+  DBuf<V4<T>>* src[7] = {&pos_, &vel_, &hydro_, &dots_, &bdots_, &hydro2_, &grav_};
+  for (int c = 0; c < 7; ++c) {
+    if (adj_end_[c].n < (size_t)nd_) adj_end_[c].alloc(nd_);
+    d2d(adj_end_[c].p, src[c]->p, (size_t)nd_, s_);
+  }
+  if (adj_end_small_.n < 32) adj_end_small_.alloc(32);
+  d2d(adj_end_small_.p, dt_cur_.p, 1, s_);
+  d2d(adj_end_small_.p + 1, stats6_.p, 6, s_);
+  d2d(adj_end_small_.p + 7, force6_.p, 6, s_);
+  d2d(adj_end_small_.p + 13, diag_.p, 19, s_);
+  adj_end_iter_ = iteration_;
+  adj_end_time_ = global_time_;
+}
+
+template <class T>
+void Engine<T>::adj_put_back_end() {
+  DBuf<V4<T>>* dst[7] = {&pos_, &vel_, &hydro_, &dots_, &bdots_, &hydro2_, &grav_};
+  for (int c = 0; c < 7; ++c) d2d(dst[c]->p, adj_end_[c].p, (size_t)nd_, s_);
+  d2d(dt_cur_.p, adj_end_small_.p, 1, s_);
+  d2d(stats6_.p, adj_end_small_.p + 1, 6, s_);
+  d2d(force6_.p, adj_end_small_.p + 7, 6, s_);
+  d2d(diag_.p, adj_end_small_.p + 13, 19, s_);
+  iteration_ = adj_end_iter_;
+  global_time_ = adj_end_time_;
+}
+
+template <class T>
 void Engine<T>::adj_capture(EvalSlot& e) {
   const size_t nd = (size_t)nd_, tot = (size_t)ngb_tot_;
   auto grow = [](auto& b, size_t n) {
@@ -718,6 +746,7 @@ double Engine<T>::adjoint_forward(int steps, int ckpt_every) {
     adj_K_ = K;
     adj_k_ = ck;
     adj_iter0_ = (int)iteration_;
+    adj_t0_ = global_time_;
     ModeGuard g{adj_mode_, adj_capture_};
     adj_mode_ = kAdjRecord;
     init();
@@ -728,6 +757,7 @@ double Engine<T>::adjoint_forward(int steps, int ckpt_every) {
       if (s % ck == 0 && s < K) adj_save(ckb, s / ck);
     }
     d2d(adj_final_vel_.p, vel_.p, nd, s_);
+    adj_keep_end();
     adj_ready_ = true;
     return adj_loss();
   }
@@ -756,6 +786,7 @@ void Engine<T>::adjoint_backward(double* gx, double* gv, double* gu) {
       adj_load(ckb, sg);
       CS_HIP(hipMemcpyAsync(dt_cur_, &adj_dt_[first - 1], 8, hipMemcpyHostToDevice, s_));
       iteration_ = adj_iter0_ + first - 1;
+      global_time_ = adj_t0_;   // (time does not enter the map)
       for (int s = first; s <= last; ++s) {
         adj_capture_ = s - first;
         step();
@@ -788,6 +819,8 @@ void Engine<T>::adjoint_backward(double* gx, double* gv, double* gu) {
     if (gx) CS_HIP(hipMemcpy(gx, adj_out_.p, 3 * n * 8, hipMemcpyDeviceToHost));
     if (gv) CS_HIP(hipMemcpy(gv, adj_out_.p + 3 * n, 3 * n * 8, hipMemcpyDeviceToHost));
     if (gu) CS_HIP(hipMemcpy(gu, adj_out_.p + 6 * n, n * 8, hipMemcpyDeviceToHost));
+    adj_put_back_end();
+    sync();
   }
 }
 
@@ -820,6 +853,7 @@ double Engine<T>::adjoint_replay(const double* x0, const double* v0, const doubl
     ModeGuard g{adj_mode_, adj_capture_};
     adj_mode_ = kAdjReplay;
     iteration_ = adj_iter0_;
+    global_time_ = adj_t0_;
     init();
     for (int s = 1; s <= adj_K_; ++s) step();
     return adj_loss();
@@ -831,6 +865,8 @@ double Engine<T>::adjoint_replay(const double* x0, const double* v0, const doubl
   template void Engine<T>::adj_load(DBuf<V4<T>>* [4], long);                    \
   template double Engine<T>::adj_loss();                                        \
   template void Engine<T>::adj_capture(EvalSlot&);                              \
+  template void Engine<T>::adj_keep_end();                                      \
+  template void Engine<T>::adj_put_back_end();                                  \
   template void Engine<T>::adj_restore(const EvalSlot&);                        \
   template void Engine<T>::adj_forces_backward(bool);                           \
   template double Engine<T>::adjoint_forward(int, int);                         \

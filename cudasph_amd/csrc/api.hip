@@ -191,6 +191,10 @@ int sph_engine_run(void* e, int steps) {
 int sph_engine_sync(void* e) {
   return guard([&] { E(e)->sync(); return 0; }, -1);
 }
+int sph_engine_statistics(void* e, double* out) {
+  return guard([&] { E(e)->statistics(out); return (int)SPH_STAT_COUNT; }, -1);
+}
+
 int sph_engine_stats(void* e, double* out11) {
   return guard([&] { E(e)->stats(out11); return 0; }, -1);
 }

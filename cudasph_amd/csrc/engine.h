@@ -29,6 +29,8 @@ class EngineBase {
   virtual void run(int steps) = 0;
   virtual void sync() = 0;
   virtual void stats(double* out11) = 0;
+  // everything system_statistics.cpp:4-82 reports, SPH_STAT_COUNT values
+  virtual void statistics(double* out) = 0;
   virtual int timers(const char** names, double* ms, long* calls, int max_n) = 0;
   virtual void reset_timers() = 0;
   virtual const sph_params& params() const = 0;

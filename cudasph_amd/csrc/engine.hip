@@ -110,6 +110,8 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
   partials_.alloc(reduce_partials(nd_) + 64);
   force6_.alloc(6);
   stats6_.alloc(6);
+  diag_part_.alloc(diag_partials(n_));
+  diag_.alloc(19);
   key_hi_.alloc(nd_);
   key_lo_.alloc(nd_);
   if (mixed()) {
@@ -144,6 +146,7 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
   CS_HIP(hipMemsetAsync(dt_cur_, 0, 8, s_));
   CS_HIP(hipMemsetAsync(dt_next_, 0, 8, s_));
   CS_HIP(hipMemsetAsync(stats6_, 0, 48, s_));
+  CS_HIP(hipMemsetAsync(diag_, 0, diag_.bytes(), s_));
   CS_HIP(hipMemsetAsync(force6_, 0, 48, s_));
 
   // identity hash and far padding records (read_state.cpp:70-89)
@@ -237,6 +240,8 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "iteration") return iteration_;
   if (name == "ngb_tot") return (double)ngb_tot_;
   if (name == "overflow_reruns") return overflow_reruns_;
+  if (name == "phantom_reads") return phantom_reads_;
+  if (name == "phantom_ngb") return phantom_ngb_;
   if (name == "exact_rsqrt") return exact_rsqrt_;
   if (name == "i_order") return i_order_;
   if (name == "grav_split_only") return grav_split_only_;
@@ -309,6 +314,45 @@ void Engine<T>::stats(double* o) {
   o[3] = s6[0] + s6[1] + s6[2];
   o[4] = std::sqrt(s6[3] * s6[3] + s6[4] * s6[4] + s6[5] * s6[5]);
   for (int k = 0; k < 6; ++k) o[5 + k] = f6[k];
+}
+
+// system_statistics.cpp:4-82 (values of the last STATS pass, which runs where
+// the reference's does: after add_gravity, before the corrector; the list
+// counters of the last find_ngb)
+template <class T>
+void Engine<T>::statistics(double* o) {
+  stats(o);
+  double s6[6], d[19];
+  CS_HIP(hipMemcpy(s6, stats6_, 48, hipMemcpyDeviceToHost));
+  CS_HIP(hipMemcpy(d, diag_, sizeof d, hipMemcpyDeviceToHost));
+  for (int c = 0; c < 3; ++c) {
+    o[SPH_STAT_JX + c] = s6[3 + c];
+    o[SPH_STAT_XMIN + c] = d[c];
+    o[SPH_STAT_XMAX + c] = d[3 + c];
+  }
+  const double m = d[6];
+  o[SPH_STAT_MTOT] = m;
+  double r2 = 0, v2 = 0;
+  for (int c = 0; c < 3; ++c) {
+    o[SPH_STAT_XCOM + c] = m != 0 ? d[7 + c] / m : 0.0;
+    o[SPH_STAT_VXCOM + c] = m != 0 ? d[10 + c] / m : 0.0;
+    r2 += o[SPH_STAT_XCOM + c] * o[SPH_STAT_XCOM + c];
+    v2 += o[SPH_STAT_VXCOM + c] * o[SPH_STAT_VXCOM + c];
+  }
+  o[SPH_STAT_RCM] = std::sqrt(r2);
+  o[SPH_STAT_VCM] = std::sqrt(v2);
+  const double mean = d[13] / n_, msq = d[14] / n_;
+  o[SPH_STAT_NGB_MEAN] = mean;
+  o[SPH_STAT_NGB_SIGMA] = std::sqrt(std::max(0.0, msq - mean * mean));
+  o[SPH_STAT_NGB_MIN] = d[16];
+  o[SPH_STAT_NGB_MAX] = d[17];
+  o[SPH_STAT_COUNT_MISMATCH] = d[15];
+  o[SPH_STAT_MAX_COUNT] = d[18];
+  o[SPH_STAT_PHANTOM_READS] = phantom_reads_;
+  o[SPH_STAT_PHANTOM_NGB] = phantom_ngb_;
+  o[SPH_STAT_OVERFLOW_RERUNS] = overflow_reruns_;
+  o[SPH_STAT_ITERATION] = iteration_;
+  o[SPH_STAT_TIME] = global_time_;
 }
 
 // ---------------------------------------------------------------------------
@@ -411,11 +455,13 @@ void Engine<T>::find() {
       launch_find_ngb<T>(pos_, tree_, hash_, ngb_offset_, attempt == 0 ? n_ngb_.p : ngb_count_.p, ngb_count_,
                          list_, flags_, nd_, wp, 0, s_);
     }
-    int fl = 0;
-    CS_HIP(hipMemcpyAsync(&h_small_[0], flags_, 4, hipMemcpyDeviceToHost, s_));
+    int fl[3] = {0, 0, 0};
+    CS_HIP(hipMemcpyAsync(&h_small_[2], flags_, 12, hipMemcpyDeviceToHost, s_));
     sync();
-    std::memcpy(&fl, &h_small_[0], 4);
-    if (!fl) return;
+    std::memcpy(fl, &h_small_[2], 12);
+    phantom_reads_ = fl[1];
+    phantom_ngb_ = fl[2];
+    if (!fl[0]) return;
     ++overflow_reruns_;
     launch_exclusive_scan(ngb_count_, ngb_offset_, nd_, 1, scan_tmp_, s_);
     check_total();
@@ -636,10 +682,12 @@ void Engine<T>::stats_pass() {
   if constexpr (kF32) {
     if (mixed()) {
       launch_stats_mixed(pos64_, vel64_, u64_, grav_, n_, partials_, stats6_, s_);
+      launch_diagnostics<double>(pos64_, vel64_, n_ngb_, ngb_count_, n_, diag_part_, diag_, s_);
       return;
     }
   }
   launch_stats<T>(pos_, vel_, hydro_, grav_, n_, partials_, stats6_, s_);
+  launch_diagnostics<T>(pos_, vel_, n_ngb_, ngb_count_, n_, diag_part_, diag_, s_);
 }
 
 // sort_bodies.cpp:119-186 on the GPU: 3*bits-bit Peano-Hilbert keys sorted

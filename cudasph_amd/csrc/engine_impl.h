@@ -53,6 +53,7 @@ class Engine final : public EngineBase {
   }
   void sync() override { CS_HIP(hipStreamSynchronize(s_)); }
   void stats(double* out11) override;
+  void statistics(double* out) override;
   int timers(const char** names, double* ms, long* calls, int max_n) override {
     sync();
     return timers_.list(names, ms, calls, max_n);
@@ -149,6 +150,15 @@ class Engine final : public EngineBase {
   void adj_capture(EvalSlot& e);
   void adj_restore(const EvalSlot& e);
   DBuf<V4<T>> adj_init_pos_, adj_init_vel_, adj_init_hyd_, adj_final_vel_;
+  // the state adjoint_forward leaves (pos vel hydro dots bdots hydro2 grav;
+  // dt, the stats, force and diagnostics sums; step count and time), put
+  // back when adjoint_backward ends so that step() continues the recorded run
+  DBuf<V4<T>> adj_end_[7];
+  DBuf<double> adj_end_small_;
+  int adj_end_iter_ = 0;
+  double adj_end_time_ = 0, adj_t0_ = 0;
+  void adj_keep_end();
+  void adj_put_back_end();
   DBuf<V4<T>> ax_, av_, aa_;       // carried adjoints of the state: x, v, (a, udot)
   DBuf<double> au_;                // carried adjoint of u
   DBuf<V4<T>> xs_, vs_, hb1_, hb2_;  // one evaluation: x, v, (rho, P, u), (divv, curl, Omega, Psi)
@@ -170,6 +180,7 @@ class Engine final : public EngineBase {
   DBuf<int> n_ngb_, ngb_count_, ngb_offset_, list_, owner_, tr_start_, tr_end_, flags_, scan_tmp_, sort_tmp_;
   DBuf<T> acc_;
   DBuf<double> dt_cur_, dt_next_, partials_, force6_, stats6_;
+  DBuf<double> diag_part_, diag_;   // system statistics beyond the energies (launch_diagnostics)
   DBuf<uint32_t> key_hi_, key_lo_;
   // mixed form
   DBuf<D4> pos64_, vel64_, old64_;
@@ -203,6 +214,7 @@ class Engine final : public EngineBase {
   DBuf<int> tree_work_;   // allocated on first use of the device build
   long ngb_tot_ = 0;
   int overflow_reruns_ = 0;
+  int phantom_reads_ = 0, phantom_ngb_ = 0;   // walks of the last find_ngb that read node n+1 / took it
 };
 
 }  // namespace cudasph

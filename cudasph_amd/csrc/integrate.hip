@@ -383,6 +383,109 @@ void launch_stats_mixed(const D4* pos64, const D4* vel64, const double* u64, con
 }
 
 // ---------------------------------------------------------------------------
+// The rest of system_statistics.cpp:19-64: bounding box, mass-weighted sums
+// for the centre of mass, and the h-solve counts n_ngb (min, max, sum, sum of
+// squares), plus the particles whose list length ngb_count differs from
+// n_ngb and the longest list.  Each block writes kDiag values to its partials
+// row; one block folds the rows in a fixed order (sums of integers are exact
+// in doubles, min and max in any order), so the result is deterministic.
+constexpr int kDiag = 19;
+enum { kDgMin = 0, kDgMax = 3, kDgM = 6, kDgMx = 7, kDgMv = 10, kDgSum = 13, kDgSq = 14, kDgMis = 15,
+       kDgNmin = 16, kDgNmax = 17, kDgCmax = 18 };
+
+__device__ __forceinline__ bool diag_is_min(int c) { return c < kDgMax || c == kDgNmin; }
+__device__ __forceinline__ bool diag_is_max(int c) { return (c >= kDgMax && c < kDgM) || c == kDgNmax || c == kDgCmax; }
+
+__device__ __forceinline__ double diag_fold(int c, double a, double b) {
+  if (diag_is_min(c)) return fmin(a, b);
+  if (diag_is_max(c)) return fmax(a, b);
+  return a + b;
+}
+
+// fold v[kDiag] over the block: within each wavefront by shuffles, then the
+// wavefront results in LDS in wavefront order; thread 0 holds the result
+__device__ __forceinline__ void block_diag(double v[kDiag]) {
+  __shared__ double s[kRedBlock / 64][kDiag];
+  for (int c = 0; c < kDiag; ++c)
+    for (int o = 32; o > 0; o >>= 1) v[c] = diag_fold(c, v[c], __shfl_xor(v[c], o, 64));
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0)
+    for (int c = 0; c < kDiag; ++c) s[w][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int c = 0; c < kDiag; ++c) {
+      double a = s[0][c];
+      for (int k = 1; k < kRedBlock / 64; ++k) a = diag_fold(c, a, s[k][c]);
+      v[c] = a;
+    }
+}
+
+__device__ __forceinline__ void diag_identity(double v[kDiag]) {
+  for (int c = 0; c < kDiag; ++c) v[c] = diag_is_min(c) ? 1e300 : diag_is_max(c) ? -1e300 : 0.0;
+}
+
+template <class P>
+__global__ __launch_bounds__(kRedBlock) void k_diagnostics(const V4<P>* __restrict__ pos,
+                                                            const V4<P>* __restrict__ vel,
+                                                            const int* __restrict__ n_ngb,
+                                                            const int* __restrict__ ngb_count, int n,
+                                                            double* __restrict__ partials) {
+  const int i = blockIdx.x * kRedBlock + threadIdx.x;
+  double v[kDiag];
+  diag_identity(v);
+  if (i < n) {
+    const V4<P> p = pos[i], u = vel[i];
+    const double m = u.w, x = p.x, y = p.y, z = p.z;
+    v[kDgMin] = v[kDgMax] = x;
+    v[kDgMin + 1] = v[kDgMax + 1] = y;
+    v[kDgMin + 2] = v[kDgMax + 2] = z;
+    v[kDgM] = m;
+    v[kDgMx] = m * x;
+    v[kDgMx + 1] = m * y;
+    v[kDgMx + 2] = m * z;
+    v[kDgMv] = m * (double)u.x;
+    v[kDgMv + 1] = m * (double)u.y;
+    v[kDgMv + 2] = m * (double)u.z;
+    const int k = n_ngb[i], c = ngb_count[i];
+    v[kDgSum] = k;
+    v[kDgSq] = (double)k * k;
+    v[kDgMis] = c != k;
+    v[kDgNmin] = v[kDgNmax] = k;
+    v[kDgCmax] = c;
+  }
+  block_diag(v);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < kDiag; ++c) partials[(size_t)kDiag * blockIdx.x + c] = v[c];
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_diag_final(const double* __restrict__ partials, int nb,
+                                                           double* __restrict__ out) {
+  double v[kDiag];
+  diag_identity(v);
+  for (int b = threadIdx.x; b < nb; b += kRedBlock)
+    for (int c = 0; c < kDiag; ++c) v[c] = diag_fold(c, v[c], partials[(size_t)kDiag * b + c]);
+  block_diag(v);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < kDiag; ++c) out[c] = v[c];
+}
+
+long diag_partials(int n) { return (long)kDiag * (ceil_div(n, kRedBlock) + 1); }
+
+template <class P>
+void launch_diagnostics(const V4<P>* pos, const V4<P>* vel, const int* n_ngb, const int* ngb_count, int n,
+                        double* partials, double* out, hipStream_t s) {
+  const int nb = ceil_div(n, kRedBlock);
+  hipLaunchKernelGGL(k_diagnostics<P>, dim3(nb), dim3(kRedBlock), 0, s, pos, vel, n_ngb, ngb_count, n, partials);
+  CS_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kRedBlock), 0, s, partials, nb, out);
+  CS_CHECK_LAUNCH();
+}
+template void launch_diagnostics<float>(const F4*, const F4*, const int*, const int*, int, double*, double*,
+                                        hipStream_t);
+template void launch_diagnostics<double>(const D4*, const D4*, const int*, const int*, int, double*, double*,
+                                         hipStream_t);
+
+// ---------------------------------------------------------------------------
 // timestep: per-block minimum, then one block's minimum (exact in any order)
 __device__ __forceinline__ void block_min(double v, double* out) {
   __shared__ double s[kRedBlock];

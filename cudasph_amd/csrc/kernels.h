@@ -88,7 +88,8 @@ void launch_reaction(const V4<T>* pos, V4<T>* bdots, const T* ngb_acc, const KV*
 void launch_reaction_mixed(const F4* pos, const F4* pos_lo, F4* bdots, const float* ngb_acc, const KV* pairs,
                            const int* owner, const int* start, const int* end, int n, hipStream_t s);
 // Peano-Hilbert keys on the bounding cube (sort_bodies.cpp:121-163); keys
-// are 3*bits wide, written as (hi,lo) 32-bit halves
+// are 3*bits wide, written as (hi,lo) 32-bit halves; scratch holds
+// 6 * (ceil(n / 256) + 1) reals of type T
 template <class T>
 void launch_ph_keys(const V4<T>* pos, int n, int bits, uint32_t* key_hi, uint32_t* key_lo, int* scratch,
                     hipStream_t s);
@@ -141,6 +142,15 @@ void launch_stats(const V4<T>* pos, const V4<T>* vel, const V4<T>* hydro, const 
 void launch_stats_mixed(const D4* pos64, const D4* vel64, const double* u64, const F4* grav, int n,
                         double* partials, double* out6, hipStream_t s);
 long reduce_partials(int n);   // doubles needed for partial sums of n items (x6 components)
+// bounding box, mass and mass-weighted position and velocity sums, and the
+// h-solve counts n_ngb (min, max, sum, sum of squares) of the n real
+// particles, the number whose list length ngb_count differs from n_ngb, and
+// the longest list: 19 doubles into out (layout: integrate.hip, kDg*);
+// partials holds diag_partials(n) doubles
+long diag_partials(int n);
+template <class P>
+void launch_diagnostics(const V4<P>* pos, const V4<P>* vel, const int* n_ngb, const int* ngb_count, int n,
+                        double* partials, double* out, hipStream_t s);
 // permutation gather of a 4-record array: dst[t] = src[perm[t]]
 template <class R>
 void launch_gather(const R* src, R* dst, const KV* perm, int n, hipStream_t s);

@@ -376,8 +376,9 @@ __device__ unsigned long long ph_key_dev(int x, int y, int z, int bits) {
   return key;
 }
 
-// bounding box: per-block min/max of x,y,z, then the key kernel folds the
-// block results (min and max are exact in any order)
+// bounding box (sort_bodies.cpp:121-134): per-block min/max of x,y,z, then
+// one block folds the block results into box[6 nb .. 6 nb + 5] (min and max
+// are exact in any order)
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_bbox(const V4<T>* __restrict__ pos, int n, T* __restrict__ out) {
   __shared__ T s[6][kBlock];
@@ -397,18 +398,34 @@ __global__ __launch_bounds__(kBlock) void k_bbox(const V4<T>* __restrict__ pos, 
 }
 
 template <class T>
+__global__ __launch_bounds__(kBlock) void k_bbox_fold(T* __restrict__ box, int nb) {
+  __shared__ T s[6][kBlock];
+  T v[6];
+  for (int c = 0; c < 6; ++c) v[c] = box[c];   // block 0 always exists
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+    for (int c = 0; c < 3; ++c) v[c] = fmin(v[c], box[6 * b + c]);
+    for (int c = 3; c < 6; ++c) v[c] = fmax(v[c], box[6 * b + c]);
+  }
+  for (int c = 0; c < 6; ++c) s[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      for (int c = 0; c < 3; ++c) s[c][threadIdx.x] = fmin(s[c][threadIdx.x], s[c][threadIdx.x + off]);
+      for (int c = 3; c < 6; ++c) s[c][threadIdx.x] = fmax(s[c][threadIdx.x], s[c][threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) box[6 * nb + threadIdx.x] = s[threadIdx.x][0];
+}
+
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_ph_keys(const V4<T>* __restrict__ pos, int n, int bits,
-                                                     const T* __restrict__ box, int n_box, uint32_t* key_hi,
+                                                     const T* __restrict__ box, uint32_t* key_hi,
                                                      uint32_t* key_lo) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   T lo[3], hi[3];
   for (int c = 0; c < 3; ++c) { lo[c] = box[c]; hi[c] = box[3 + c]; }
-  for (int b = 1; b < n_box; ++b)
-    for (int c = 0; c < 3; ++c) {
-      lo[c] = fmin(lo[c], box[6 * b + c]);
-      hi[c] = fmax(hi[c], box[6 * b + 3 + c]);
-    }
   const T size = fmax(hi[2] - lo[2], fmax(hi[1] - lo[1], hi[0] - lo[0]));
   const T fac = T(1.0 / (double)size * (double)(1LL << bits));
   const V4<T> p = pos[i];
@@ -425,7 +442,10 @@ void launch_ph_keys(const V4<T>* pos, int n, int bits, uint32_t* key_hi, uint32_
   T* box = reinterpret_cast<T*>(scratch);
   hipLaunchKernelGGL(k_bbox<T>, dim3(nb), dim3(kBlock), 0, s, pos, n, box);
   CS_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ph_keys<T>, dim3(nb), dim3(kBlock), 0, s, pos, n, bits, box, nb, key_hi, key_lo);
+  hipLaunchKernelGGL(k_bbox_fold<T>, dim3(1), dim3(kBlock), 0, s, box, nb);
+  CS_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ph_keys<T>, dim3(nb), dim3(kBlock), 0, s, pos, n, bits, box + 6 * (size_t)nb, key_hi,
+                     key_lo);
   CS_CHECK_LAUNCH();
 }
 template void launch_ph_keys<float>(const F4*, int, int, uint32_t*, uint32_t*, int*, hipStream_t);

@@ -24,10 +24,14 @@ namespace cudasph {
 // every node with 2p <= n_nodes, which for even n_nodes includes the
 // nonexistent node n+1; the tree buffer holds a zero record there, which is
 // what the reference's out-of-range texture fetch returns.  fix_phantom drops
-// that push instead.  Returns the number of nodes visited.
+// that push instead.  Returns the number of nodes visited.  With `phantom`
+// given, bit 0 of *phantom is set when the walk reads node n+1 and bit 1 when
+// that read passes the distance test (a spurious visit of particle 0 when the
+// zero record sits there, n = n_dev).
 template <class T, class Visit>
 __device__ __forceinline__ int kd_walk(const V4<T> q, const V4<T>* __restrict__ tree, int n_nodes,
-                                       int fix_phantom, int* __restrict__ stk, Visit&& visit) {
+                                       int fix_phantom, int* __restrict__ stk, Visit&& visit,
+                                       int* phantom = nullptr) {
   int top = 0;
   stk[0] = 1;
   int n_walk = 0;
@@ -38,6 +42,7 @@ __device__ __forceinline__ int kd_walk(const V4<T> q, const V4<T>* __restrict__ 
   while (top >= 0) {
     ++n_walk;
     int id = stk[top * kBlock];
+    const bool ph = phantom && id == n_nodes + 1;
     const V4<T> node = tree[id];
     const int bits = node_bits(node.w);
     const int split = (bits >> 24) & 3;
@@ -52,6 +57,7 @@ __device__ __forceinline__ int kd_walk(const V4<T> q, const V4<T>* __restrict__ 
     const T dx = qx - node.x, dy = qy - node.y, dz = qz - node.z;
     const T ds2 = dx * dx + dy * dy + dz * dz;
     if (ds2 < r2) visit(ds2, bits & ~(3 << 24));
+    if (ph) *phantom |= ds2 < r2 ? 3 : 1;
   }
   return n_walk;
 }
@@ -162,8 +168,10 @@ __global__ __launch_bounds__(kBlock) void k_solve_range(V4<T>* __restrict__ pos,
 // dev_find_ngb.cu:75-87.  Capacity of particle i's slot is
 // n_norm(cap_count[i], 4) (the layout of sph_accelerations.cpp:58-64); at most
 // that many ids are written, the true count goes to count_out[i], and
-// flags[0] is raised when a count exceeds its slot.  With clamp set,
-// count_out is clamped to the slot (for callers that own the offsets).
+// flags[0] is raised when a count exceeds its slot; flags[1] and flags[2]
+// count the walks that read node n+1 and that took it as a neighbour.  With
+// clamp set, count_out is clamped to the slot (for callers that own the
+// offsets).  flags holds at least 3 ints, zeroed by the caller.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_find_ngb(const V4<T>* __restrict__ pos, const V4<T>* __restrict__ tree,
                                                       KV* __restrict__ hash, const int* __restrict__ offset,
@@ -182,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_find_ngb(const V4<T>* __restrict__ p
   // wavefront's lanes write 64 different lists, so a 4-byte store per id
   // would dirty a cache line per lane per id
   const bool vec = (off & 3) == 0 && (reinterpret_cast<uintptr_t>(list) & 15) == 0;
-  int c = 0;
+  int c = 0, ph = 0;
   int b0 = 0, b1 = 0, b2 = 0;   // the current group's first three ids
   const int nw = kd_walk<T>(q, tree, wp.n_nodes, wp.fix_phantom, s_stack + threadIdx.x, [&](T, int id) {
     const int r = c & 3;
@@ -198,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_find_ngb(const V4<T>* __restrict__ p
       list[off + c] = id;
     }
     ++c;
-  });
+  }, &ph);
   if (vec) {   // the last, partial group (slots past the count stay untouched)
     const int g = c & ~3, r = c & 3;
     if (r > 0 && g < cap) {
@@ -208,6 +216,8 @@ __global__ __launch_bounds__(kBlock) void k_find_ngb(const V4<T>* __restrict__ p
     }
   }
   hash[t].key = (uint32_t)nw;
+  if (ph & 1) atomicAdd(flags + 1, 1);   // walks that read node n+1 (rare: a few per tree)
+  if (ph & 2) atomicAdd(flags + 2, 1);   // ... and took it as a neighbour
   if (c > cap) {
     atomicOr(flags, 1);
     if (clamp) c = cap;

@@ -206,6 +206,35 @@ int    sph_engine_run(void* e, int steps);
 int    sph_engine_sync(void* e);
 /* T U W E |J| Fx Fy Fz Tx Ty Tz of the last STATS / ADD_GRAVITY passes */
 int    sph_engine_stats(void* e, double* out11);
+
+/* Indices of sph_engine_statistics' output. */
+enum {
+  SPH_STAT_T = 0, SPH_STAT_U, SPH_STAT_W, SPH_STAT_E, SPH_STAT_J,    /* as sph_engine_stats */
+  SPH_STAT_FX, SPH_STAT_FY, SPH_STAT_FZ, SPH_STAT_TX, SPH_STAT_TY, SPH_STAT_TZ,
+  SPH_STAT_JX, SPH_STAT_JY, SPH_STAT_JZ,                              /* angular momentum      */
+  SPH_STAT_XMIN, SPH_STAT_YMIN, SPH_STAT_ZMIN,                        /* computational box     */
+  SPH_STAT_XMAX, SPH_STAT_YMAX, SPH_STAT_ZMAX,
+  SPH_STAT_MTOT,                                                      /* total mass            */
+  SPH_STAT_XCOM, SPH_STAT_YCOM, SPH_STAT_ZCOM,                        /* centre of mass        */
+  SPH_STAT_VXCOM, SPH_STAT_VYCOM, SPH_STAT_VZCOM,                     /* its velocity          */
+  SPH_STAT_RCM, SPH_STAT_VCM,                                         /* |Rcm|, |Vcm|          */
+  SPH_STAT_NGB_MEAN, SPH_STAT_NGB_SIGMA, SPH_STAT_NGB_MIN, SPH_STAT_NGB_MAX,  /* h-solve counts */
+  SPH_STAT_COUNT_MISMATCH,   /* particles whose list length differs from the h-solve count */
+  SPH_STAT_MAX_COUNT,        /* longest neighbour list                                      */
+  SPH_STAT_PHANTOM_READS,    /* walks of the last find_ngb that read node n+1               */
+  SPH_STAT_PHANTOM_NGB,      /* ... and took its record as a neighbour (only when n = n_dev) */
+  SPH_STAT_OVERFLOW_RERUNS,  /* find_ngb re-walks after a slot overflow, since creation     */
+  SPH_STAT_ITERATION, SPH_STAT_TIME,
+  SPH_STAT_COUNT
+};
+/* system_statistics.cpp:4-82: what the reference prints every step --
+ * energies, box, centre of mass, |J| and the neighbour-count mean, sigma, min
+ * and max (over the real particles' h-solve counts n_ngb, as there) -- plus
+ * the list diagnostics above, from the last STATS pass (in step(): after the
+ * gravity is added, before the corrector, as step.cpp:125).  Sums are in
+ * double (the reference sums the energies in float).  out receives
+ * SPH_STAT_COUNT doubles.  Returns SPH_STAT_COUNT, or -1. */
+int    sph_engine_statistics(void* e, double* out);
 /* Per-kernel GPU milliseconds accumulated since the last reset, measured with
  * HIP events on the engine's stream; names[i] is a static string. */
 int    sph_engine_timers(void* e, const char** names, double* ms, long* calls, int max_n);
@@ -222,11 +251,15 @@ void   sph_engine_reset_timers(void* e);
  *                        *loss receives L.
  *  sph_adjoint_backward: dL/dx0, dL/dv0 (n x 3 doubles each) and dL/du0 (n
  *                        doubles) of the recorded run into host arrays (any may
- *                        be NULL); recomputes each checkpoint segment.
+ *                        be NULL); recomputes each checkpoint segment, then
+ *                        puts back the state, dt, step count and time that
+ *                        sph_adjoint_forward left, so a following step()
+ *                        continues the recorded run.
  *  sph_adjoint_replay:   L of the recorded (frozen) map from another initial
  *                        state: x0, v0 (n x 3) and u0 (n) host arrays, NULL =
  *                        the recorded run's own.  Leaves the engine in the
- *                        replayed final state.
+ *                        replayed final state (time and step count as at the
+ *                        end of the recorded run).
  * All return 0, or -1 with sph_last_error() set. */
 int    sph_adjoint_forward(void* e, int steps, int ckpt_every, double* loss);
 int    sph_adjoint_backward(void* e, double* grad_x, double* grad_v, double* grad_u);

@@ -18,6 +18,11 @@ LIB_PATH = HERE / "liboracle.so"
 (TREE, SOLVE, OFFSETS, FIND, DENSITY, EOS, HYDRO, REACTION, GRAVITY, ADD_GRAVITY, TIMESTEP,
  PREDICT, CORRECT, STATS, PH_SORT, SPH_ACC, INIT, STEP) = range(18)
 STATS_KEYS = ("T", "U", "W", "E", "J", "Fx", "Fy", "Fz", "Tx", "Ty", "Tz")
+STATISTICS_KEYS = STATS_KEYS + (
+    "Jx", "Jy", "Jz", "xmin", "ymin", "zmin", "xmax", "ymax", "zmax", "m_tot",
+    "xcom", "ycom", "zcom", "vxcom", "vycom", "vzcom", "Rcm", "Vcm",
+    "ngb_mean", "ngb_sigma", "ngb_min", "ngb_max", "count_mismatch", "max_count",
+    "phantom_reads", "phantom_ngb", "overflow_reruns", "iteration", "time")
 
 
 def build() -> Path:
@@ -48,6 +53,7 @@ def lib():
         L.orc_pass.argtypes = [vp, i]
         L.orc_pass.restype = i
         L.orc_stats.argtypes = [vp, C.POINTER(C.c_double)]
+        L.orc_statistics.argtypes = [vp, C.POINTER(C.c_double)]
         L.orc_gravity_range.argtypes = [vp, i, i]
         L.orc_kernel_fns.argtypes = [i, i, vp, vp]
         L.orc_pressure.argtypes = [i, vp, d, d, d]
@@ -131,6 +137,11 @@ class Oracle:
         out = (C.c_double * 11)()
         lib().orc_stats(self._h, out)
         return dict(zip(STATS_KEYS, list(out)))
+
+    def statistics(self):
+        out = (C.c_double * len(STATISTICS_KEYS))()
+        lib().orc_statistics(self._h, out)
+        return dict(zip(STATISTICS_KEYS, list(out)))
 
 
 def kernel_fns(bits, u):

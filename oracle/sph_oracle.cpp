@@ -216,7 +216,7 @@ void build_node(R4<T>* tree, int node, Elem<T>* d, int depth, int n) {
  * reference fetches it out of range and gets a zero node (origin, id 0,
  * split 0). */
 template <class T, class Visit>
-int walk(const R4<T>& q, const R4<T>* tree, int n_nodes, bool fix_phantom, Visit&& visit) {
+int walk(const R4<T>& q, const R4<T>* tree, int n_nodes, bool fix_phantom, Visit&& visit, int* phantom = nullptr) {
   int stack[64];
   int top = 0;
   stack[0] = 1;
@@ -224,6 +224,7 @@ int walk(const R4<T>& q, const R4<T>* tree, int n_nodes, bool fix_phantom, Visit
   while (top > -1) {
     n_walk++;
     int id = stack[top];
+    const bool ph = phantom && id == n_nodes + 1;   // bit 0: node n+1 read, bit 1: within range
     const R4<T> node = tree[id];
     const int split = (meta_bits(node.w) & (3 << 24)) >> 24;
     id *= 2;
@@ -240,6 +241,7 @@ int walk(const R4<T>& q, const R4<T>* tree, int n_nodes, bool fix_phantom, Visit
     const T dx = q.x - node.x, dy = q.y - node.y, dz = q.z - node.z;
     const T ds2 = dx * dx + dy * dy + dz * dz;
     if (ds2 < q.w * q.w) visit(ds2, meta_bits(node.w) & ~(3 << 24));
+    if (ph) *phantom |= ds2 < q.w * q.w ? 3 : 1;
   }
   return n_walk;
 }
@@ -538,8 +540,14 @@ struct Base {
   virtual void gravity_range(int i0, int i1) = 0;
   double dt = 0, global_time = 0;
   int iteration = 0, n_overflow_reruns = 0, n_count_mismatch = 0, max_count = 0;
+  int phantom_reads = 0, phantom_ngb = 0;
   long ngb_tot = 0;
   Stats st{};
+  /* the rest of system_statistics.cpp:19-64: box, mass, centre of mass and
+   * its velocity, J components, h-solve count mean / sigma / min / max */
+  double r_min[3] = {0, 0, 0}, r_max[3] = {0, 0, 0}, m_tot = 0, r_com[3] = {0, 0, 0}, v_com[3] = {0, 0, 0};
+  double Jv[3] = {0, 0, 0}, ngb_mean = 0, ngb_sigma = 0;
+  int ngb_min = 0, ngb_max = 0;
 };
 
 template <class T> struct State : Base {
@@ -642,21 +650,25 @@ template <class T> struct State : Base {
    * true count; on overflow the offsets are rebuilt from the true counts */
   void find() {
     for (int pass = 0; pass < 2; ++pass) {
-      int overflow = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(| : overflow)
+      int overflow = 0, reads = 0, took = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : overflow) reduction(+ : reads, took)
       for (int t = 0; t < nd; ++t) {
         const int i = (int)hash[2 * t + 1];
         const int off = ngb_offset[i];
         const int cap_i = ngb_offset[i + 1] - off;
-        int c = 0;
+        int c = 0, ph = 0;
         const int nw = walk<T>(pos[i], tree.data(), n, p.fix_phantom != 0, [&](T, int id) {
           if (c < cap_i) ngb_list[off + c] = id;
           c++;
-        });
+        }, &ph);
+        reads += ph & 1;
+        took += (ph >> 1) & 1;
         hash[2 * t] = (uint32_t)nw;
         ngb_count[i] = c;
         if (c > cap_i) overflow = 1;
       }
+      phantom_reads = reads;
+      phantom_ngb = took;
       if (!overflow) break;
       n_overflow_reruns++;
       offsets(ngb_count);
@@ -851,6 +863,10 @@ template <class T> struct State : Base {
   /* system_statistics.cpp:4-84, accumulated in double */
   void stats() {
     double Tk = 0, U = 0, W = 0, J[3] = {0, 0, 0};
+    double lo[3] = {double(pos[0].x), double(pos[0].y), double(pos[0].z)};
+    double hi[3] = {lo[0], lo[1], lo[2]};
+    double M = 0, mr[3] = {0, 0, 0}, mv[3] = {0, 0, 0}, sum = 0, sq = 0;
+    int kmin = n, kmax = 0;
     for (int i = 0; i < n; ++i) {
       const double m = vel[i].w;
       const double x = pos[i].x, y = pos[i].y, z = pos[i].z;
@@ -861,9 +877,29 @@ template <class T> struct State : Base {
       J[0] += m * (y * vz - z * vy);
       J[1] += m * (z * vx - x * vz);
       J[2] += m * (x * vy - y * vx);
+      const double r[3] = {x, y, z}, v[3] = {vx, vy, vz};
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = std::min(lo[c], r[c]);
+        hi[c] = std::max(hi[c], r[c]);
+        mr[c] += m * r[c];
+        mv[c] += m * v[c];
+      }
+      M += m;
+      kmin = std::min(kmin, n_ngb[i]);
+      kmax = std::max(kmax, n_ngb[i]);
+      sum += n_ngb[i];
+      sq += double(n_ngb[i]) * n_ngb[i];
     }
     st.T = Tk; st.U = U; st.W = W; st.E = Tk + U + W;
     st.J = std::sqrt(J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
+    for (int c = 0; c < 3; ++c) {
+      r_min[c] = lo[c]; r_max[c] = hi[c]; Jv[c] = J[c];
+      r_com[c] = mr[c] / M; v_com[c] = mv[c] / M;
+    }
+    m_tot = M;
+    ngb_mean = sum / n;
+    ngb_sigma = std::sqrt(sq / n - ngb_mean * ngb_mean);
+    ngb_min = kmin; ngb_max = kmax;
   }
 
   /* sort_bodies.cpp:119-186: PH keys on the bounding cube, stable sort,
@@ -972,6 +1008,8 @@ double orc_get_scalar(void* h, const char* name) {
   if (s == "overflow_reruns") return b->n_overflow_reruns;
   if (s == "count_mismatch") return b->n_count_mismatch;
   if (s == "max_count") return b->max_count;
+  if (s == "phantom_reads") return b->phantom_reads;
+  if (s == "phantom_ngb") return b->phantom_ngb;
   return NAN;
 }
 void orc_set_scalar(void* h, const char* name, double v) {
@@ -987,6 +1025,28 @@ void orc_stats(void* h, double* o) {
   const Stats& s = static_cast<Base*>(h)->st;
   o[0] = s.T; o[1] = s.U; o[2] = s.W; o[3] = s.E; o[4] = s.J;
   for (int k = 0; k < 3; ++k) { o[5 + k] = s.F[k]; o[8 + k] = s.Tq[k]; }
+}
+
+void orc_statistics(void* h, double* o) {
+  const Base* b = static_cast<Base*>(h);
+  orc_stats(h, o);
+  double r = 0, v = 0;
+  for (int c = 0; c < 3; ++c) {
+    o[11 + c] = b->Jv[c];
+    o[14 + c] = b->r_min[c];
+    o[17 + c] = b->r_max[c];
+    o[21 + c] = b->r_com[c];
+    o[24 + c] = b->v_com[c];
+    r += b->r_com[c] * b->r_com[c];
+    v += b->v_com[c] * b->v_com[c];
+  }
+  o[20] = b->m_tot;
+  o[27] = std::sqrt(r);
+  o[28] = std::sqrt(v);
+  o[29] = b->ngb_mean; o[30] = b->ngb_sigma; o[31] = b->ngb_min; o[32] = b->ngb_max;
+  o[33] = b->n_count_mismatch; o[34] = b->max_count;
+  o[35] = b->phantom_reads; o[36] = b->phantom_ngb;
+  o[37] = b->n_overflow_reruns; o[38] = b->iteration; o[39] = b->global_time;
 }
 
 void orc_kernel_fns(int real_bits, int n, const double* u, double* out) {

@@ -64,6 +64,9 @@ void   orc_gravity_range(void* h, int i0, int i1);
 /* statistics of the last ORC_STATS pass: T U W E |J| Fx Fy Fz Tx Ty Tz
  * (force and torque from the last ORC_ADD_GRAVITY) */
 void   orc_stats(void* h, double* out11);
+/* everything system_statistics.cpp:4-82 reports plus the list counters, in
+ * the layout of the engine's sph_engine_statistics (40 doubles) */
+void   orc_statistics(void* h, double* out40);
 
 /* stateless helpers for unit tests */
 void   orc_kernel_fns(int real_bits, int n, const double* u, double* out9);

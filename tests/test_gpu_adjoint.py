@@ -155,3 +155,22 @@ def test_adjoint_errors(cs):
         f.adjoint_backward()
     with pytest.raises(RuntimeError, match="ckpt_every"):
         f.adjoint_forward(2, 0)
+
+
+def test_step_after_backward_continues_the_recorded_run(cs):
+    """adjoint_backward puts back the state adjoint_forward left: a step()
+    afterwards gives the same bits, time and step count as init() plus K + 1
+    plain steps."""
+    n, K = 2048, 4
+    print(f"step after backward n={n} K={K}", flush=True)
+    a, _, _ = fp64_engine(cs, n, seed=13)
+    a.adjoint_forward(K, 2)
+    a.adjoint_backward()
+    a.step()
+    b, _, _ = fp64_engine(cs, n, seed=13)
+    b.init()
+    b.step(K + 1)
+    for name in ("pos", "vel", "hydro", "bdots", "grav"):
+        assert np.array_equal(a.get(name)[:n], b.get(name)[:n]), name
+    for key in ("dt", "global_time", "iteration"):
+        assert a.scalar(key) == b.scalar(key), key

@@ -102,6 +102,63 @@ def test_plain_steps_bit_exact(cs, orc, cfg, n):
         assert se[key] == pytest.approx(so[key], rel=1e-12, abs=1e-14), key
 
 
+@pytest.mark.parametrize("cfg,n", CASES + [(1, 16000)])
+def test_statistics_against_oracle(cs, orc, cfg, n):
+    """Everything system_statistics.cpp:4-82 reports (box, centre of mass,
+    energies, neighbour-count mean/sigma/min/max) and the list counters
+    (count/list mismatches, longest list, walks that read node n+1 and took
+    it) after two bit-exact steps: integers and the box exactly, sums to
+    1e-12 relative.  n = 16384 = n_dev puts the zero record at node n+1."""
+    print(f"statistics cfg={cfg} n={n}", flush=True)
+    e, o, _ = make_pair(cs, orc, cfg, n)
+    e.set_scalar("exact_rsqrt", 1)
+    e.init()
+    o.init()
+    for _ in range(2):
+        e.step()
+        o.step()
+    se, so = e.statistics(), o.statistics()
+    exact = ("xmin", "ymin", "zmin", "xmax", "ymax", "zmax", "ngb_min", "ngb_max", "count_mismatch",
+             "max_count", "phantom_reads", "phantom_ngb", "overflow_reruns", "iteration", "time")
+    for key in exact:
+        assert se[key] == so[key], (key, se[key], so[key])
+    for key in ("T", "U", "W", "E", "J", "Jx", "Jy", "Jz", "m_tot", "xcom", "ycom", "zcom", "vxcom", "vycom",
+                "vzcom", "Rcm", "Vcm", "ngb_mean", "ngb_sigma"):
+        assert se[key] == pytest.approx(so[key], rel=1e-12, abs=1e-15), key
+    if n == e.nd:
+        assert so["phantom_reads"] > 0
+    target = {1: 32, 2: 50, 3: 64}[cfg]
+    assert abs(se["ngb_mean"] - target) < 0.15 * target
+
+
+def test_mixed_statistics_follow_the_fp64_state(cs):
+    """The mixed form's statistics come from its FP64 positions and
+    velocities and its own lists (velocities are compared loosely: the
+    statistics see them before the corrector, as the reference's do, and the
+    corrector moves the centre-of-mass velocity only by the residual total
+    force)."""
+    n = 16384
+    print(f"mixed statistics n={n}", flush=True)
+    e = cs.make_engine("mixed", 3, n, 9)
+    e.init()
+    e.step()
+    s = e.statistics()
+    x = e.get("pos64")[:n].astype(np.float64)
+    v = e.get("vel64")[:n].astype(np.float64)
+    k = e.get("n_ngb")[:n].astype(np.float64)
+    m = v[:, 3]
+    for c, ax in enumerate("xyz"):
+        assert (s[ax + "min"], s[ax + "max"]) == (x[:, c].min(), x[:, c].max())
+        assert s[ax + "com"] == pytest.approx((m * x[:, c]).sum() / m.sum(), rel=1e-12, abs=1e-15)
+        assert s["v" + ax + "com"] == pytest.approx((m * v[:, c]).sum() / m.sum(), abs=1e-8)
+    assert (s["ngb_min"], s["ngb_max"]) == (k.min(), k.max())
+    assert s["ngb_mean"] == pytest.approx(k.mean(), rel=1e-14)
+    assert s["ngb_sigma"] == pytest.approx(k.std(), rel=1e-9)
+    cnt = e.get("ngb_count")[:n]
+    assert s["count_mismatch"] == np.count_nonzero(cnt != e.get("n_ngb")[:n])
+    assert s["max_count"] == cnt.max()
+
+
 def test_plummer_ten_steps_lists_bit_exact(cs, orc):
     """The Plummer sphere over 10 steps (Courant 0.3), neighbour lists compared
     slot by slot after every step, and the state after the last."""
