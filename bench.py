@@ -16,7 +16,12 @@ Extra objects on the JSON line:
                 157.3 TFLOP/s FP32 peak; traffic = HBM bytes per launch from
                 the committed rocprofv3 counter summary when present
   cpu_baseline  the CPU oracle (reference arithmetic, OpenMP over particles)
-                on the host cores, on a bounded sample of the same workload
+                on the host cores, on a bounded sample of the same workload,
+                and on one core
+  scale         the scale workload (turbulent uniform sphere, N = 4,194,304):
+                5 mixed steps with per-pass SPH GB/s and the gravity FP32
+                fraction, the same steps with the far field, and one
+                reverse-mode step of the FP64 form
 """
 from __future__ import annotations
 
@@ -138,11 +143,24 @@ def traffic_from_profile(kernel_prefix):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(cs, n, seed, gravity_sample):
-    """Oracle FP32 step on the host: SPH passes and integration on all N, the
-    N^2 gravity on `gravity_sample` i-particles against all N j, scaled."""
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo) and logical CPU count."""
+    name = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return name, os.cpu_count()
+
+
+def cpu_step(cs, n, seed, threads, gravity_sample):
+    """One oracle FP32 step on `threads` host threads: SPH passes and
+    integration on all N, the N^2 gravity on `gravity_sample` i-particles
+    against all N j, scaled to N.  Returns (seconds per step, parts)."""
     import oracle
-    threads = min(16, os.cpu_count() or 1)
     p = cs.preset_params(3, n)
     p.n_threads = threads
     pos, vel, hyd = cs.generate_ic(3, n, seed)
@@ -161,13 +179,50 @@ def cpu_baseline(cs, n, seed, gravity_sample):
     o.run_pass(oracle.PREDICT)
     o.run_pass(oracle.CORRECT)
     t_int = time.perf_counter() - t0
-    step = t_sph + t_grav + t_int
+    return t_sph + t_grav + t_int, (t_sph, t_grav, t_int, p.n_dev / gravity_sample)
+
+
+def cpu_share():
+    """Host CPUs this process may use: the CPUs in its affinity mask, capped by
+    the cgroup's CPU quota (cpu.max) and by OMP_NUM_THREADS when they are set.
+    Returns (threads, affinity count, quota or None)."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    threads = affinity
+    if quota:
+        threads = min(threads, max(1, int(quota)))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return threads, affinity, quota
+
+
+def cpu_baseline(cs, n, seed, gravity_sample, gravity_sample_1core):
+    """The CPU oracle (the reference's FP32 arithmetic, OpenMP over
+    particles) on every host core this process may use, and on one core."""
+    threads, affinity, quota = cpu_share()
+    step, (t_sph, t_grav, t_int, scale) = cpu_step(cs, n, seed, threads, gravity_sample)
+    progress(f"cpu baseline on {threads} threads: {step:.1f} s per step")
+    step1, (s1, g1, i1, scale1) = cpu_step(cs, n, seed, 1, gravity_sample_1core)
+    progress(f"cpu baseline on 1 thread: {step1:.1f} s per step")
+    model, logical = cpu_model()
     return {
         "value": n / step, "unit": "particle-updates/s", "cores": threads, "kind": "port",
-        "sample": (f"one plain-FP32 step of the same {n}-particle workload on {threads} threads: "
-                   f"tree+h-solve+lists+density+EOS+hydro+reaction on all particles ({t_sph:.2f} s), "
-                   f"direct gravity for {gravity_sample} i-particles against all j scaled by "
-                   f"{p.n_dev / gravity_sample:.0f} ({t_grav:.1f} s), integration ({t_int:.3f} s)"),
+        "cpu": model, "logical_cpus": logical, "affinity_cpus": affinity, "cgroup_cpu_quota": quota,
+        "sample": (f"one plain-FP32 step of the same {n}-particle workload on {threads} threads (the "
+                   f"CPUs this process may use: affinity mask, cgroup quota, OMP_NUM_THREADS): tree+h-solve+lists+density+EOS+hydro+reaction on all "
+                   f"particles ({t_sph:.2f} s), direct gravity for {gravity_sample} i-particles against all j "
+                   f"scaled by {scale:.0f} ({t_grav:.1f} s), integration ({t_int:.3f} s)"),
+        "one_core": {"value": n / step1, "unit": "particle-updates/s", "cores": 1,
+                     "sample": (f"the same step on 1 thread: SPH passes on all particles ({s1:.2f} s), gravity "
+                                f"for {gravity_sample_1core} i-particles scaled by {scale1:.0f} ({g1:.1f} s), "
+                                f"integration ({i1:.3f} s)")},
     }
 
 
@@ -285,7 +340,8 @@ def scale_leg(cs, n, steps, seed):
                          "gravity_ms": t_far["gravity"][0] / max(t_far["gravity"][1], 1),
                          "energy_vs_pairwise": {"definition": "|E_far - E_pairwise| / |E(0)| / steps after the run",
                                                 "value": abs(e_far - e_pair) / abs(e0) / steps}}}
-    print(json.dumps({"scale_progress": "mixed steps done", "ms_per_step": out["ms_per_step"]}), flush=True)
+    progress(f"scale: {steps} mixed steps at N={n}, {out['ms_per_step']:.0f} ms/step; far field "
+             f"{out['far_field']['ms_per_step']:.0f} ms/step")
     p = cs.preset_params(5, n)
     pos, vel, hyd = cs.generate_ic(5, n, seed)
     f = cs.Engine("fp64", p)
@@ -298,6 +354,7 @@ def scale_leg(cs, n, steps, seed):
     f.adjoint_backward()
     t_bwd = time.perf_counter() - t0
     f.close()
+    progress(f"scale: one FP64 adjoint step, forward {t_fwd:.1f} s, backward {t_bwd:.1f} s")
     out["adjoint_step"] = {"form": "fp64", "steps": 1, "loss": loss, "forward_s": t_fwd, "backward_s": t_bwd,
                            "adjoint_over_forward": t_bwd / t_fwd}
     return out
@@ -314,6 +371,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-gravity-sample", type=int, default=4096)
+    ap.add_argument("--cpu-gravity-sample-1core", type=int, default=512)
     ap.add_argument("--i-order", type=int, default=0, help="0: memory order, 1: kd-tree heap order")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-far", action="store_true",
@@ -326,6 +384,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="run only the reverse-mode leg")
     ap.add_argument("--scale", action="store_true",
                     help="run only the scale workload (N = 4,194,304: 5 mixed steps + one adjoint step)")
+    ap.add_argument("--no-scale", action="store_true",
+                    help="skip the scale workload, which the default run reports in a `scale` object")
     ap.add_argument("--scale-n", type=int, default=4194304)
     ap.add_argument("--scale-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -447,8 +507,11 @@ def main():
         adj = adjoint_leg(cs, args.adjoint_n, args.adjoint_steps, args.adjoint_ckpt, args.fd_dirs, args.seed)
         line["adjoint_over_forward"] = adj["adjoint_over_forward"]
         line["adjoint"] = adj
+    if not args.no_scale:
+        line["scale"] = scale_leg(cs, args.scale_n, args.scale_steps, args.seed)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(cs, n, args.seed, args.cpu_gravity_sample)
+        line["cpu_baseline"] = cpu_baseline(cs, n, args.seed, args.cpu_gravity_sample,
+                                            args.cpu_gravity_sample_1core)
     print(json.dumps(line))
 
 

@@ -26,6 +26,10 @@
 // The per-pair j-side adjoints are written to one record per list slot and
 // summed per j through the forward's sorted (neighbour, slot) pairs, in
 // ascending slot order: the backward is deterministic, like the reaction pass.
+// With adj_implicit_h = 1 the ranges are not held: each evaluation adds the
+// derivative of its h-solve (see k_adj_range_hydro below), so the gradient is
+// that of the map whose ranges are solved from the positions (timesteps and
+// counts still recorded).
 #include <stdexcept>
 
 #include "engine_impl.h"
@@ -557,6 +561,192 @@ __global__ __launch_bounds__(kBlock) void k_adj_set_z(D4v* __restrict__ dst, con
   dst[i].z = src[i];
 }
 
+// ---- the ranges' share of the gradient (engine scalar adj_implicit_h) -----
+// The frozen map holds each evaluation's ranges r_i; the engine's map solves
+// them from the positions: F_i(r_i, x) = sum_j ig(2 |x_ij| / r_i) - target = 0
+// (dev_solve_range.cu:17-89).  Its derivative adds, per evaluation,
+//   x_bar += r_bar_i dr_i/dx,  dr_i/dx = -(dF_i/dx) / (dF_i/dr_i),
+// with r_bar_i = dL/dr_i through everything the evaluation computes from r_i:
+// the density outputs (rho, Omega, Psi, div v, |curl v| of i) and the hydro
+// pair terms of i's list (u = |x_ij| / h_i, the h^-5 and h^-2 factors, and the
+// Balsara switch's 1e-4 c/h of i and, through the viscosity of pair (i, j),
+// of j).  Forward derivatives in r_i (h = r / 2, du/dr = -u / r) contracted
+// with the adjoints the reverse pass already holds.
+
+// hydro pairs: r_bar_i of i's own terms into rbar[i]; the j-side part (the
+// Balsara factor of j) of every slot into pb[k * kPair]
+__global__ __launch_bounds__(kBlock) void k_adj_range_hydro(const D4v* __restrict__ pos, const D4v* __restrict__ vel,
+                                                             const D4v* __restrict__ hydro,
+                                                             const D4v* __restrict__ dots,
+                                                             const D4v* __restrict__ abar,
+                                                             const int* __restrict__ offset,
+                                                             const int* __restrict__ count,
+                                                             const int* __restrict__ list, double* __restrict__ rbar,
+                                                             double* __restrict__ pb, int n, double alpha,
+                                                             double beta) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const D4v bp = pos[i], bv = vel[i], hd = hydro[i], dd = dots[i], ai = abar[i];
+  const double mi = bv.w, h = 0.5 * bp.w, h2 = h * h, h5 = h2 * h2 * h;
+  const double rho = hd.x, P = hd.y, prho2 = P / (rho * rho);
+  const double Om = dd.z, Ps = dd.w, Di = dd.x, Ci = dd.y;
+  const double ci = sqrt(5.0 / 3.0 * P / rho);
+  const double deni = (fabs(Di) + Ci) + 0.0001 * ci / h;
+  const double bi = fabs(Di) / deni;
+  const double dh = 0.5;                                   // dh / dr_i
+  const double dbi = deni > 0.0 ? fabs(Di) * (0.0001 * ci / (h * h) * dh) / (deni * deni) : 0.0;
+  const double dih5 = -5.0 / (h5 * h) * dh, dih2 = -2.0 / (h2 * h) * dh;   // d(h^-5), d(h^-2)
+  const double K = prho2 * Om + 0.5 * Ps;
+  double rb = 0.0;
+  const int off = offset[i], cnt = count[i];
+  for (int k = off; k < off + cnt; ++k) {
+    const int j = list[k];
+    double* out = pb + (size_t)k * kPair;
+    if (j == i) {
+      out[0] = 0.0;
+      continue;
+    }
+    const D4v pj = pos[j], vj = vel[j], hj = hydro[j], dj = dots[j], aj = abar[j];
+    const double mj = vj.w;
+    const double rx = bp.x - pj.x, ry = bp.y - pj.y, rz = bp.z - pj.z;
+    const double dvx = bv.x - vj.x, dvy = bv.y - vj.y, dvz = bv.z - vj.z;
+    const double ds = sqrt(rx * rx + ry * ry + rz * rz);
+    const double u = ds / h;
+    const double drdv = rx * dvx + ry * dvy + rz * dvz;
+    double Pi = 0.0, dPi_i = 0.0, dPi_j = 0.0;
+    if (drdv < 0.0) {
+      const double rhoj = hj.x, Pj = hj.y, prho2j = Pj / (rhoj * rhoj);
+      const double cj = sqrt(5.0 / 3.0 * Pj / rhoj);
+      const double hjw = 0.5 * pj.w;
+      const double mu0 = drdv / ((ci + cj) * ds);
+      const double denj = (fabs(dj.x) + dj.y) + 0.0001 * cj / hjw;
+      const double bj = fabs(dj.x) / denj;
+      const double dbj = denj > 0.0 ? fabs(dj.x) * (0.0001 * cj / (hjw * hjw) * 0.5) / (denj * denj) : 0.0;
+      const double mu = mu0 * (bi + bj), S = prho2 + prho2j;
+      Pi = S * (-alpha * mu + beta * (mu * mu));
+      const double dPi_dmu = S * (-alpha + 2.0 * beta * mu);
+      dPi_i = dPi_dmu * mu0 * dbi;
+      dPi_j = dPi_dmu * mu0 * dbj;
+    }
+    const double w1 = kern_dw<double>(u), g = kern_dig<double>(u);
+    const double du = -u / h * dh;
+    const double dw1 = u > 0.0 ? u * ddw_u(u) * du : 0.0;
+    const double dg = u > 0.0 ? u * ddig_u(u) * du : 0.0;
+    const double pv = prho2 + 0.5 * Pi;
+    const double dpvw = pv * (dw1 / h5 + w1 * dih5);       // d(pv w1 / h^5) at fixed pv
+    const double dgh = dg / h2 + g * dih2;                  // d(g / h^2)
+    const double dA_i = 0.5 * dPi_i * w1 / h5 + dpvw - K / mj * dgh;
+    const double dQ_i = mj * (0.5 * dPi_i * w1 / h5 + dpvw) - prho2 * Om * dgh;
+    const double dA_j = 0.5 * dPi_j * w1 / h5, dQ_j = mj * dA_j;
+    const double Ab = (mi * aj.x - mj * ai.x) * rx + (mi * aj.y - mj * ai.y) * ry + (mi * aj.z - mj * ai.z) * rz;
+    const double Qb = ai.w * drdv;
+    rb += Ab * dA_i + Qb * dQ_i;
+    out[0] = Ab * dA_j + Qb * dQ_j;
+  }
+  rbar[i] = rb;
+}
+
+// rbar[j] += the j-side parts of the slots naming j (ascending slot order)
+__global__ __launch_bounds__(kBlock) void k_adj_range_gather(const KV* __restrict__ pairs,
+                                                             const int* __restrict__ start,
+                                                             const int* __restrict__ end,
+                                                             const double* __restrict__ pb,
+                                                             double* __restrict__ rbar, int n) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int p = start[j]; p < end[j]; ++p) s += pb[(size_t)pairs[p].value * kPair];
+  rbar[j] += s;
+}
+
+// density outputs' part of r_bar_i, then the h-solve's implicit derivative:
+// x_bar_i += lam sum_j (4 / r^2) dig(u_ij) x_ij, x_bar_j -= lam (4 / r^2)
+// dig(u_ij) x_ij (per slot into pb[k * kPair + 0..2]), lam = r_bar_i r_i /
+// sum_j digdh(u_ij)  (dF/dr = -sum_j digdh(u) / r, dF/dx_i = sum_j (4 / r^2)
+// dig(u) x_ij)
+__global__ __launch_bounds__(kBlock) void k_adj_range_density(const D4v* __restrict__ pos,
+                                                               const D4v* __restrict__ vel,
+                                                               const D4v* __restrict__ hydro,
+                                                               const D4v* __restrict__ dots,
+                                                               const D4v* __restrict__ hb1,
+                                                               const D4v* __restrict__ hb2,
+                                                               const int* __restrict__ offset,
+                                                               const int* __restrict__ count,
+                                                               const int* __restrict__ list,
+                                                               const double* __restrict__ rbar, D4v* __restrict__ xs,
+                                                               double* __restrict__ pb, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const D4v bp = pos[i], bv = vel[i];
+  const double mi = bv.w, r = bp.w, h = 0.5 * r, h3 = h * h * h, h5 = h3 * h * h;
+  const int off = offset[i], cnt = count[i];
+  double den = 0, cx = 0, cy = 0, cz = 0;   // the forward's normalisation and curl sums
+  double rho_d = 0, om_d = 0, psi_d = 0, den_d = 0, dv_d = 0, cx_d = 0, cy_d = 0, cz_d = 0, sdig = 0;
+  for (int k = off; k < off + cnt; ++k) {
+    const int j = list[k];
+    const D4v pj = pos[j], vj = vel[j];
+    const double mj = vj.w;
+    const double rx = bp.x - pj.x, ry = bp.y - pj.y, rz = bp.z - pj.z;
+    const double dvx = bv.x - vj.x, dvy = bv.y - vj.y, dvz = bv.z - vj.z;
+    const double u = 2.0 * sqrt(rx * rx + ry * ry + rz * rz) / r;
+    den -= kern_digdh<double>(u);
+    sdig += kern_digdh<double>(u);
+    const double mdw = mi * kern_dw<double>(u);
+    const double ex = ry * dvz - rz * dvy, ey = rz * dvx - rx * dvz, ez = rx * dvy - ry * dvx;
+    cx += mdw * ex;
+    cy += mdw * ey;
+    cz += mdw * ez;
+    if (u > 0.0) {   // d/dr of each term: f'(u) du/dr, du/dr = -u / r
+      const double du = -u / r;
+      rho_d += mj * u * kern_dw<double>(u) * du;
+      om_d += mj * u * ddwdh_u(u) * du;
+      psi_d += mj * u * ddphidh_u(u) * du;
+      den_d -= ddigdh(u) * du;
+      const double mddw = mi * u * ddw_u(u) * du;
+      dv_d += mddw * (rx * dvx + ry * dvy + rz * dvz);
+      cx_d += mddw * ex;
+      cy_d += mddw * ey;
+      cz_d += mddw * ez;
+    }
+  }
+  const D4v hd = hydro[i], dd = dots[i], b1 = hb1[i], b2 = hb2[i];
+  const double rho = hd.x;
+  // outputs' derivatives in r (dh/dr = 1/2); div v and |curl v| at fixed rho,
+  // rho's own derivative carries their dependence through rho_bar below
+  const double drho = rho_d / h3 - 1.5 * rho / h;
+  const double rhob = b1.x - b2.x * dd.x / rho - b2.y * dd.y / rho;
+  double rb = rbar[i] + rhob * drho;
+  rb += b2.x * (dv_d / (rho * h5) - 2.5 * dd.x / h);
+  const double cn = sqrt(cx * cx + cy * cy + cz * cz);
+  if (cn > 0.0) rb += b2.y * ((cx * cx_d + cy * cy_d + cz * cz_d) / (cn * rho * h5) - 2.5 * dd.y / h);
+  if (den != 0.0) {
+    rb += b2.z * (om_d / (h3 * den) - dd.z * (1.5 / h + den_d / den));
+    rb += b2.w * (psi_d / (h * den) - dd.w * (0.5 / h + den_d / den));
+  }
+  const double lam = sdig != 0.0 ? rb * r / sdig : 0.0;
+  const double c4 = 4.0 / (r * r) * lam;
+  double gx = 0, gy = 0, gz = 0;
+  for (int k = off; k < off + cnt; ++k) {
+    const int j = list[k];
+    double* out = pb + (size_t)k * kPair;
+    const D4v pj = pos[j];
+    const double rx = bp.x - pj.x, ry = bp.y - pj.y, rz = bp.z - pj.z;
+    const double u = 2.0 * sqrt(rx * rx + ry * ry + rz * rz) / r;
+    const double f = j == i ? 0.0 : c4 * kern_dig<double>(u);
+    gx += f * rx;
+    gy += f * ry;
+    gz += f * rz;
+    out[0] = -f * rx;
+    out[1] = -f * ry;
+    out[2] = -f * rz;
+  }
+  D4v x = xs[i];
+  x.x += gx;
+  x.y += gy;
+  x.z += gz;
+  xs[i] = x;
+}
+
 inline dim3 grid(int n) { return dim3(ceil_div(n, kBlock)); }
 
 template <class X>
@@ -700,6 +890,22 @@ void Engine<T>::adj_forces_backward(bool seed_rho) {
     hipLaunchKernelGGL(k_adj_gather, grid(n_), dim3(kBlock), 0, s_, pairs_.p, tr_start_.p, tr_end_.p, pairbuf_.p,
                        xs_.p, vs_.p, hb1_.p, hb2_.p, n_, 6);
     CS_CHECK_LAUNCH();
+    if (adj_implicit_h_) {   // the ranges' dependence on the positions (the h-solve)
+      if (adj_rbar_.n < (size_t)nd_) adj_rbar_.alloc(nd_);
+      hipLaunchKernelGGL(k_adj_range_hydro, grid(n_), dim3(kBlock), 0, s_, pos_.p, vel_.p, hydro_.p, dots_.p, aa_.p,
+                         ngb_offset_.p, ngb_count_.p, list_.p, adj_rbar_.p, pairbuf_.p, n_, (double)p_.av_alpha,
+                         (double)p_.av_beta);
+      CS_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_adj_range_gather, grid(n_), dim3(kBlock), 0, s_, pairs_.p, tr_start_.p, tr_end_.p,
+                         pairbuf_.p, adj_rbar_.p, n_);
+      CS_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_adj_range_density, grid(n_), dim3(kBlock), 0, s_, pos_.p, vel_.p, hydro_.p, dots_.p,
+                         hb1_.p, hb2_.p, ngb_offset_.p, ngb_count_.p, list_.p, adj_rbar_.p, xs_.p, pairbuf_.p, n_);
+      CS_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_adj_gather, grid(n_), dim3(kBlock), 0, s_, pairs_.p, tr_start_.p, tr_end_.p, pairbuf_.p,
+                         xs_.p, vs_.p, hb1_.p, hb2_.p, n_, 3);
+      CS_CHECK_LAUNCH();
+    }
     ts.t.end(s_);
     ts.t.begin("adjoint_gravity", s_);
     const double eps = (double)p_.grav_eps;

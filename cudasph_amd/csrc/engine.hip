@@ -96,7 +96,7 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
   ngb_count_.alloc(nd_);
   ngb_offset_.alloc(nd_ + 1);
   list_.alloc(cap_);
-  owner_.alloc(cap_);
+  owner_.alloc(nd_);   // per-j counts of the reaction transpose
   acc_.alloc(cap_);
   pairs_.alloc(cap_);
   pairs_alt_.alloc(cap_);
@@ -247,6 +247,10 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_split_only") return grav_split_only_;
   if (name == "grav_unit_mass") return grav_unit_mass_;
   if (name == "grav_expand") return grav_expand_;
+  if (name == "grav_mfma") return grav_mfma_;
+  if (name == "solve_cache") return solve_cache_on_;
+  if (name == "adj_free_ranges") return adj_free_ranges_;
+  if (name == "adj_implicit_h") return adj_implicit_h_;
   if (name == "grav_far") return grav_far_;
   if (name == "grav_far_ratio") return grav_far_ratio_;
   if (name == "grav_far_grade") return grav_far_grade_;
@@ -280,6 +284,14 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     grav_unit_mass_ = (int)v;
   } else if (name == "grav_expand") {
     grav_expand_ = (int)v;
+  } else if (name == "grav_mfma") {
+    grav_mfma_ = (int)v;
+  } else if (name == "solve_cache") {
+    solve_cache_on_ = (int)v;
+  } else if (name == "adj_free_ranges") {
+    adj_free_ranges_ = (int)v;
+  } else if (name == "adj_implicit_h") {
+    adj_implicit_h_ = (int)v;
   } else if (name == "grav_far") {
     grav_far_ = (int)v;
   } else if (name == "grav_far_ratio") {
@@ -403,14 +415,19 @@ void Engine<T>::tree() {
 template <class T>
 void Engine<T>::solve() {
   const size_t rec = (size_t)adj_f_ * nd_;
-  if (adj_mode_ >= kAdjReplay) {   // the recorded ranges and counts of this evaluation
+  // the recorded ranges and counts of this evaluation (a replay with
+  // adj_free_ranges set solves them afresh instead, for measuring what the
+  // frozen ranges leave out of the gradient)
+  if (adj_mode_ >= kAdjReplay && !(adj_mode_ == kAdjReplay && adj_free_ranges_)) {
     launch_set_range<T>(pos_, adj_range_.p + rec, nd_, s_);
     CS_HIP(hipMemcpyAsync(n_ngb_, adj_nngb_.p + rec, (size_t)nd_ * 4, hipMemcpyDeviceToDevice, s_));
     return;
   }
   {
     TimerScope ts(timers_, "solve_range", s_);
-    launch_solve_range<T>(pos_, n_ngb_, hash_, tree_, nd_, walk_params(), s_);
+    if (solve_cache_on_ && solve_cache_.n == 0) solve_cache_.alloc((size_t)nd_ * kSolveCacheCap);
+    launch_solve_range<T>(pos_, n_ngb_, hash_, tree_, nd_, walk_params(), solve_cache_on_ ? solve_cache_.p : nullptr,
+                          kSolveCacheCap, s_);
   }
   if (adj_mode_ == kAdjRecord) {
     launch_get_range<T>(pos_, adj_range_.p + rec, nd_, s_);
@@ -513,20 +530,14 @@ void Engine<T>::hydro() {
   }
 }
 
-// reaction scatter (sph_accelerations.cpp:104-115) as a gather: sort the list
-// slots by target j (stable, so each j sees its sources in ascending slot =
-// ascending i order, the reference's serial order) and let each j sum its
-// own reactions
-// list slots sorted by neighbour id (stable): for each j, the slots k whose
-// neighbour is j, in ascending k
+// reaction scatter (sph_accelerations.cpp:104-115) as a gather: the list
+// slots grouped by target j, each group in ascending slot = ascending i order
+// (the reference's serial order), and each j sums its own reactions
 template <class T>
 void Engine<T>::build_transpose() {
   check_total();
-  launch_reaction_pairs(ngb_offset_, ngb_count_, list_, pairs_, owner_, nd_, s_);
-  launch_radix_sort(pairs_, pairs_alt_, ngb_tot_, bits_for(nd_), sort_tmp_, s_);
-  CS_HIP(hipMemsetAsync(tr_start_, 0, tr_start_.bytes(), s_));
-  CS_HIP(hipMemsetAsync(tr_end_, 0, tr_end_.bytes(), s_));
-  launch_reaction_bounds(pairs_, ngb_tot_, nd_, tr_start_, tr_end_, s_);
+  launch_reaction_transpose(ngb_offset_, ngb_count_, list_, nd_, owner_, tr_start_, tr_end_, pairs_, pairs_alt_,
+                            scan_tmp_, s_);
 }
 
 template <class T>
@@ -535,11 +546,11 @@ void Engine<T>::reaction() {
   build_transpose();
   if constexpr (kF32) {
     if (mixed()) {
-      launch_reaction_mixed(pos_, pos_lo_, bdots_, acc_, pairs_, owner_, tr_start_, tr_end_, n_, s_);
+      launch_reaction_mixed(pos_, pos_lo_, bdots_, acc_, pairs_, tr_start_, tr_end_, n_, s_);
       return;
     }
   }
-  launch_reaction<T>(pos_, bdots_, acc_, pairs_, owner_, tr_start_, tr_end_, n_, s_);
+  launch_reaction<T>(pos_, bdots_, acc_, pairs_, tr_start_, tr_end_, n_, s_);
 }
 
 template <class T>
@@ -616,6 +627,7 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
                            n_, p_.grav_mode,
                            p_.grav_eps,
                            grav_split_only_ | (grav_unit_mass_ ? 0 : 2) | (grav_expand_ ? 0 : 4) |
+                               (grav_mfma_ ? 128 : 0) |
                                (grav_far_ ? 16 : 0) | (grav_far_ && grav_far_half_ ? 64 : 0),
                            gs);
       return;

@@ -131,6 +131,9 @@ class Engine final : public EngineBase {
   void adj_load(DBuf<V4<T>>* src[4], long slot);
   double adj_loss();
   int adj_mode_ = kAdjOff, adj_K_ = 0, adj_k_ = 1, adj_f_ = 0, adj_iter0_ = 0;
+  int adj_free_ranges_ = 0;   // sph_adjoint_replay re-solves the ranges (timesteps stay recorded)
+  int adj_implicit_h_ = 0;    // sph_adjoint_backward adds the h-solve's derivative
+  DBuf<double> adj_rbar_;     // per-particle range adjoint of one evaluation
   bool adj_ready_ = false;
   std::vector<double> adj_dt_;                       // timestep set by force evaluation f = 0..K
   DBuf<double> adj_range_;                           // ranges of evaluation f, (K+1) x n_dev
@@ -202,6 +205,7 @@ class Engine final : public EngineBase {
   int grav_split_only_ = 0;
   int grav_unit_mass_ = 1;    // mixed gravity: unit-mass pair body when every mass is equal
   int grav_expand_ = 1;       // mixed gravity: expanded r^2 in the equal-mass tile body
+  int grav_mfma_ = 0;         // ... with its products and sums on the matrix cores
   int grav_far_ = 0;          // mixed gravity: order-4 multipole far field for distant sub-tiles
   double grav_far_ratio_ = 8;  // ... taken beyond this many radii of a 16384-record cell
   int grav_far_half_ = 1;      // ... also the 32-record halves of a sub-tile not taken whole
@@ -212,6 +216,12 @@ class Engine final : public EngineBase {
   // 0: on the host, bit-identical to the reference's quicksort build
   int tree_gpu_ = 0;
   DBuf<int> tree_work_;   // allocated on first use of the device build
+  // h-solve candidate cache: kSolveCacheCap distances per thread, allocated on
+  // first use (1: on, the default; 0: every evaluation walks the tree -- the
+  // same bits either way)
+  static constexpr int kSolveCacheCap = 192;
+  int solve_cache_on_ = 1;
+  DBuf<T> solve_cache_;
   long ngb_tot_ = 0;
   int overflow_reruns_ = 0;
   int phantom_reads_ = 0, phantom_ngb_ = 0;   // walks of the last find_ngb that read node n+1 / took it

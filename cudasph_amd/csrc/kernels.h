@@ -17,9 +17,11 @@ struct WalkParams {
 struct EosConst;  // sph_math.h
 
 // ---- sph_passes.hip ----
+// cache: n_dev * cap reals of per-thread candidate distances (nullptr: each
+// evaluation walks the tree); the results are the same bits either way
 template <class T>
 void launch_solve_range(V4<T>* pos, int* n_ngb, const KV* hash, const V4<T>* tree, int n_dev,
-                        const WalkParams& wp, hipStream_t s);
+                        const WalkParams& wp, T* cache, int cap, hipStream_t s);
 template <class T>
 void launch_find_ngb(const V4<T>* pos, const V4<T>* tree, KV* hash, const int* offset, const int* cap_count,
                      int* count_out, int* list, int* flags, int n_dev, const WalkParams& wp, int clamp,
@@ -72,21 +74,22 @@ void launch_exclusive_scan(const int* in, int* out, long n, int mode, int* scrat
 // radix_scratch_ints(n) ints.
 long radix_scratch_ints(long n);
 void launch_radix_sort(KV* data, KV* alt, long n, int bits, int* scratch, hipStream_t s);
-// list slots -> (key = neighbour j, value = slot k) for every slot of the
-// n particles' ranges (unused slot tails get key n); owner[k] = i
-void launch_reaction_pairs(const int* offset, const int* count, const int* list, KV* pairs, int* owner, int n,
-                           hipStream_t s);
-// per-j [start,end) of the sorted pairs (keys >= n skipped)
-void launch_reaction_bounds(const KV* pairs, long n_pairs, int n, int* start, int* end, hipStream_t s);
+// the reaction lists: for each of the n particles j, the list slots k
+// naming j in ascending k, as pairs {key = owner i, value = k} in
+// pairs[start[j], end[j]) (end[j] = start[j + 1]; start holds n + 1 ints).
+// tcnt holds n ints of scratch, alt as many pairs as pairs, scan_scratch
+// scan_scratch_ints(n) ints.
+void launch_reaction_transpose(const int* offset, const int* count, const int* list, int n, int* tcnt,
+                               int* start, int* end, KV* pairs, KV* alt, int* scan_scratch, hipStream_t s);
 // kv[t].key = keys[t]
 void launch_set_keys(KV* kv, const uint32_t* keys, int n, hipStream_t s);
-// bdots[j].xyz += ngb_acc[k] (r_i - r_j) over the sorted pairs of j, in
+// bdots[j].xyz += ngb_acc[k] (r_i - r_j) over the reaction pairs of j, in
 // ascending k (sph_accelerations.cpp:104-115)
 template <class T>
-void launch_reaction(const V4<T>* pos, V4<T>* bdots, const T* ngb_acc, const KV* pairs, const int* owner,
-                     const int* start, const int* end, int n, hipStream_t s);
+void launch_reaction(const V4<T>* pos, V4<T>* bdots, const T* ngb_acc, const KV* pairs, const int* start,
+                     const int* end, int n, hipStream_t s);
 void launch_reaction_mixed(const F4* pos, const F4* pos_lo, F4* bdots, const float* ngb_acc, const KV* pairs,
-                           const int* owner, const int* start, const int* end, int n, hipStream_t s);
+                           const int* start, const int* end, int n, hipStream_t s);
 // Peano-Hilbert keys on the bounding cube (sort_bodies.cpp:121-163); keys
 // are 3*bits wide, written as (hi,lo) 32-bit halves; scratch holds
 // 6 * (ceil(n / 256) + 1) reals of type T

@@ -1,6 +1,6 @@
 // Device-wide primitives: exclusive scan, stable LSD radix sort of
-// (key,value) records, the transpose that turns the i-side neighbour lists
-// into j-side reaction lists, and Peano-Hilbert keys.
+// (key,value) records, the counting-sort transpose that turns the i-side
+// neighbour lists into j-side reaction lists, and Peano-Hilbert keys.
 //
 // The radix sort replaces the reference's RadixSort (radixsort.cu:50-77).
 // Each pass (8 key bits) is three launches: per-tile digit histograms,
@@ -196,47 +196,98 @@ void launch_radix_sort(KV* data, KV* alt, long n, int bits, int* scratch, hipStr
 }
 
 // ---------------------------------------------------------------------------
-// reaction transpose
-// every slot k of particle i's list range [offset[i], offset[i+1]) becomes
-// (key = neighbour id, value = k); the unused tail of a slot gets the key
-// n (sorted past every real id).  owner[k] = i.
-__global__ __launch_bounds__(kBlock) void k_reaction_pairs(const int* __restrict__ offset,
-                                                            const int* __restrict__ count,
-                                                            const int* __restrict__ list, KV* __restrict__ pairs,
-                                                            int* __restrict__ owner, int n) {
-  // one wavefront per particle: its lanes write the particle's slot run, so
-  // the stores are contiguous across the wavefront
+// reaction transpose: for every j, the list slots k that name j, in
+// ascending k (ascending owner i: the slots of i are contiguous and the
+// offsets ascend with i), as pairs {key = owner i, value = k} in
+// pairs[start[j], start[j + 1]).  A counting sort: per-j counts, their
+// exclusive scan, a scatter that places each slot in its j's bucket (in
+// whatever order the atomics resolve), and a sort of each bucket by k.
+
+// one wavefront per particle i: each real slot adds one to its j's count
+__global__ __launch_bounds__(kBlock) void k_tr_count(const int* __restrict__ offset, const int* __restrict__ count,
+                                                      const int* __restrict__ list, int* __restrict__ tcnt, int n) {
   const int i = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
-  const int off = offset[i], end = offset[i + 1], c = count[i];
-  for (int k = off + lane; k < end; k += 64) {
-    pairs[k] = KV{k - off < c ? (uint32_t)list[k] : (uint32_t)n, (uint32_t)k};
-    owner[k] = i;
+  const int off = offset[i], c = count[i];
+  for (int k = off + lane; k < off + c; k += 64) atomicAdd(&tcnt[list[k]], 1);
+}
+
+// each real slot takes a place in its j's bucket; tcnt counts down to zero
+__global__ __launch_bounds__(kBlock) void k_tr_fill(const int* __restrict__ offset, const int* __restrict__ count,
+                                                     const int* __restrict__ list, const int* __restrict__ start,
+                                                     int* __restrict__ tcnt, KV* __restrict__ pairs, int n) {
+  const int i = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const int off = offset[i], c = count[i];
+  for (int k = off + lane; k < off + c; k += 64) {
+    const int j = list[k];
+    const int p = atomicSub(&tcnt[j], 1) - 1;
+    pairs[start[j] + p] = KV{(uint32_t)i, (uint32_t)k};
   }
 }
 
-void launch_reaction_pairs(const int* offset, const int* count, const int* list, KV* pairs, int* owner, int n,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(k_reaction_pairs, dim3(ceil_div(n, kBlock / 64)), dim3(kBlock), 0, s, offset, count, list,
-                     pairs, owner, n);
+// one wavefront per bucket: a bitonic sort by k in LDS (the k are distinct)
+// for buckets of up to kTrSortMax slots; a larger bucket is ranked instead
+// (each slot's place is the number of smaller k), through `alt`
+constexpr int kTrSortMax = 512;
+__global__ __launch_bounds__(64) void k_tr_sort(KV* __restrict__ pairs, KV* __restrict__ alt,
+                                                 const int* __restrict__ start, int n) {
+  __shared__ unsigned long long s[kTrSortMax];
+  const int j = blockIdx.x, lane = threadIdx.x;
+  if (j >= n) return;
+  const int b = start[j], m = start[j + 1] - b;
+  if (m <= 1) return;
+  if (m > kTrSortMax) {
+    for (int t = lane; t < m; t += 64) {
+      const KV e = pairs[b + t];
+      int r = 0;
+      for (int q = 0; q < m; ++q) r += pairs[b + q].value < e.value;
+      alt[b + r] = e;
+    }
+    __syncthreads();
+    for (int t = lane; t < m; t += 64) pairs[b + t] = alt[b + t];
+    return;
+  }
+  int P = 2;
+  while (P < m) P <<= 1;
+  for (int t = lane; t < P; t += 64) {
+    if (t < m) {
+      const KV e = pairs[b + t];
+      s[t] = ((unsigned long long)e.value << 32) | e.key;
+    } else {
+      s[t] = ~0ull;
+    }
+  }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1)
+    for (int h = k >> 1; h > 0; h >>= 1) {
+      for (int t = lane; t < P / 2; t += 64) {
+        const int x = ((t & ~(h - 1)) << 1) | (t & (h - 1)), y = x + h;
+        const unsigned long long u = s[x], v = s[y];
+        const bool up = (x & k) == 0;
+        if ((u > v) == up) {
+          s[x] = v;
+          s[y] = u;
+        }
+      }
+      __syncthreads();
+    }
+  for (int t = lane; t < m; t += 64) pairs[b + t] = KV{(uint32_t)(s[t] & 0xffffffffu), (uint32_t)(s[t] >> 32)};
+}
+
+void launch_reaction_transpose(const int* offset, const int* count, const int* list, int n, int* tcnt,
+                               int* start, int* end, KV* pairs, KV* alt, int* scan_scratch, hipStream_t s) {
+  CS_HIP(hipMemsetAsync(tcnt, 0, (size_t)n * sizeof(int), s));
+  const dim3 waves(ceil_div(n, kBlock / 64));
+  hipLaunchKernelGGL(k_tr_count, waves, dim3(kBlock), 0, s, offset, count, list, tcnt, n);
   CS_CHECK_LAUNCH();
-}
-
-__global__ __launch_bounds__(kBlock) void k_reaction_bounds(const KV* __restrict__ pairs, long n_pairs, int n,
-                                                             int* start, int* end) {
-  const long p = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (p >= n_pairs) return;
-  const uint32_t j = pairs[p].key;
-  if (j >= (uint32_t)n) return;
-  if (p == 0 || pairs[p - 1].key != j) start[j] = (int)p;
-  if (p == n_pairs - 1 || pairs[p + 1].key != j) end[j] = (int)(p + 1);
-}
-
-void launch_reaction_bounds(const KV* pairs, long n_pairs, int n, int* start, int* end, hipStream_t s) {
-  if (n_pairs <= 0) return;
-  hipLaunchKernelGGL(k_reaction_bounds, dim3(ceil_div(n_pairs, kBlock)), dim3(kBlock), 0, s, pairs, n_pairs, n,
-                     start, end);
+  launch_exclusive_scan(tcnt, start, n, 0, scan_scratch, s);
+  CS_HIP(hipMemcpyAsync(end, start + 1, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_tr_fill, waves, dim3(kBlock), 0, s, offset, count, list, start, tcnt, pairs, n);
+  CS_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_tr_sort, dim3(n), dim3(64), 0, s, pairs, alt, start, n);
   CS_CHECK_LAUNCH();
 }
 
@@ -252,8 +303,8 @@ void launch_set_keys(KV* kv, const uint32_t* keys, int n, hipStream_t s) {
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_reaction(const V4<T>* __restrict__ pos, V4<T>* __restrict__ bdots,
                                                       const T* __restrict__ ngb_acc, const KV* __restrict__ pairs,
-                                                      const int* __restrict__ owner, const int* __restrict__ start,
-                                                      const int* __restrict__ end, int n) {
+                                                      const int* __restrict__ start, const int* __restrict__ end,
+                                                      int n) {
   const int j = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (j >= n) return;
   const V4<T> pj = pos[j];
@@ -261,17 +312,16 @@ __global__ __launch_bounds__(kBlock) void k_reaction(const V4<T>* __restrict__ p
   const int p0 = start[j], p1 = end[j];
   // software pipeline: the next slot's owner, coefficient and position are
   // requested before the current term is added (the sum keeps slot order)
-  int kn = p0 < p1 ? (int)pairs[p0].value : 0;
-  int in_ = p0 < p1 ? owner[kn] : j;
-  T an = p0 < p1 ? ngb_acc[kn] : T(0);
-  V4<T> qn = pos[in_];
+  KV e = p0 < p1 ? pairs[p0] : KV{(uint32_t)j, 0u};
+  T an = p0 < p1 ? ngb_acc[e.value] : T(0);
+  V4<T> qn = pos[e.key];
   for (int p = p0; p < p1; ++p) {
     const T acc = an;
     const V4<T> pi = qn;
     if (p + 1 < p1) {
-      kn = (int)pairs[p + 1].value;
-      in_ = owner[kn];
-      an = ngb_acc[kn];
+      e = pairs[p + 1];
+      const int in_ = (int)e.key;
+      an = ngb_acc[e.value];
       qn = pos[in_];
     }
     d.x += acc * (pi.x - pj.x);
@@ -282,16 +332,16 @@ __global__ __launch_bounds__(kBlock) void k_reaction(const V4<T>* __restrict__ p
 }
 
 template <class T>
-void launch_reaction(const V4<T>* pos, V4<T>* bdots, const T* ngb_acc, const KV* pairs, const int* owner,
-                     const int* start, const int* end, int n, hipStream_t s) {
+void launch_reaction(const V4<T>* pos, V4<T>* bdots, const T* ngb_acc, const KV* pairs, const int* start,
+                     const int* end, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_reaction<T>, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, s, pos, bdots, ngb_acc, pairs,
-                     owner, start, end, n);
+                     start, end, n);
   CS_CHECK_LAUNCH();
 }
-template void launch_reaction<float>(const F4*, F4*, const float*, const KV*, const int*, const int*, const int*,
-                                     int, hipStream_t);
-template void launch_reaction<double>(const D4*, D4*, const double*, const KV*, const int*, const int*,
-                                      const int*, int, hipStream_t);
+template void launch_reaction<float>(const F4*, F4*, const float*, const KV*, const int*, const int*, int,
+                                     hipStream_t);
+template void launch_reaction<double>(const D4*, D4*, const double*, const KV*, const int*, const int*, int,
+                                      hipStream_t);
 
 // mixed form: the same gather with split positions and compensated sums; a
 // group of 8 lanes per j, each lane taking every 8th of j's slots, partial
@@ -299,7 +349,6 @@ template void launch_reaction<double>(const D4*, D4*, const double*, const KV*, 
 __global__ __launch_bounds__(kBlock) void k_reaction_mixed(const F4* __restrict__ pos, const F4* __restrict__ lo,
                                                             F4* __restrict__ bdots, const float* __restrict__ ngb_acc,
                                                             const KV* __restrict__ pairs,
-                                                            const int* __restrict__ owner,
                                                             const int* __restrict__ start,
                                                             const int* __restrict__ end, int n) {
   constexpr int kG = 8;
@@ -315,9 +364,9 @@ __global__ __launch_bounds__(kBlock) void k_reaction_mixed(const F4* __restrict_
     s[q] = t;
   };
   for (int p = start[j] + lane; p < end[j]; p += kG) {
-    const int k = (int)pairs[p].value;
-    const int i = owner[k];
-    const float acc = ngb_acc[k];
+    const KV e = pairs[p];
+    const int i = (int)e.key;
+    const float acc = ngb_acc[e.value];
     const F4 pi = pos[i], li = lo[i];
     kadd(0, acc * ((pi.x - pj.x) + (li.x - lj.x)));
     kadd(1, acc * ((pi.y - pj.y) + (li.y - lj.y)));
@@ -341,9 +390,9 @@ __global__ __launch_bounds__(kBlock) void k_reaction_mixed(const F4* __restrict_
 }
 
 void launch_reaction_mixed(const F4* pos, const F4* pos_lo, F4* bdots, const float* ngb_acc, const KV* pairs,
-                           const int* owner, const int* start, const int* end, int n, hipStream_t s) {
+                           const int* start, const int* end, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_reaction_mixed, dim3(ceil_div(n * 8, kBlock)), dim3(kBlock), 0, s, pos, pos_lo, bdots, ngb_acc,
-                     pairs, owner, start, end, n);
+                     pairs, start, end, n);
   CS_CHECK_LAUNCH();
 }
 

@@ -259,7 +259,11 @@ void   sph_engine_reset_timers(void* e);
  *                        state: x0, v0 (n x 3) and u0 (n) host arrays, NULL =
  *                        the recorded run's own.  Leaves the engine in the
  *                        replayed final state (time and step count as at the
- *                        end of the recorded run).
+ *                        end of the recorded run).  With the engine scalar
+ *                        adj_free_ranges = 1 the replay solves the ranges and
+ *                        counts afresh at every evaluation (timesteps still
+ *                        recorded): the map whose derivative the frozen-range
+ *                        gradient approximates.
  * All return 0, or -1 with sph_last_error() set. */
 int    sph_adjoint_forward(void* e, int steps, int ckpt_every, double* loss);
 int    sph_adjoint_backward(void* e, double* grad_x, double* grad_v, double* grad_u);

@@ -140,3 +140,130 @@ def test_energy_drift_against_fp64_at_full_size(cs):
         drift = np.abs(energies[form][1:] - ref[1:]) / abs(ref[0]) / np.arange(1, len(ref))
         print(f"  {form} drift per step {drift.max():.2e}", flush=True)
         assert drift.max() < 1e-6, (form, drift)
+
+
+def sampled_fp64_gravity(orc, params, pos4, vel4, windows, width):
+    """FP64 oracle gravity (sph_grav_kernel.cu:14-111 restated in double)
+    for i in the given windows of `width` particles, against all j."""
+    n = params.n_bodies
+    o = orc.Oracle(64, params)
+    P = np.zeros((params.n_dev, 4))
+    P[:n] = pos4[:n]
+    V = np.zeros((params.n_dev, 4))
+    V[:n] = vel4[:n]
+    o.set("pos", P)
+    o.set("vel", V)
+    for i0 in windows:
+        o.gravity_range(int(i0), int(i0) + width)
+    g = o.get("grav")
+    idx = np.concatenate([np.arange(i0, i0 + width) for i0 in windows])
+    return idx, g[idx]
+
+
+def gravity_errors(g, ref):
+    """Per-particle |a - a_ref| / max(|a_ref|, 1e-3 rms |a_ref|) and the
+    potential's relative error."""
+    rms = np.sqrt((ref[:, :3] ** 2).sum(1).mean())
+    err = np.linalg.norm(g[:, :3] - ref[:, :3], axis=1) / np.maximum(np.linalg.norm(ref[:, :3], axis=1), 1e-3 * rms)
+    perr = np.abs(g[:, 3] - ref[:, 3]) / np.abs(ref[:, 3])
+    return err, perr
+
+
+@pytest.mark.parametrize("cfg", [3, 5])
+def test_gravity_per_particle_against_fp64_oracle_at_full_size(cs, orc, cfg):
+    """The headline kernel at the headline size: 2048 i-particles (16 windows
+    of 128 spread over the Peano-Hilbert order) of N = 2^20 against the FP64
+    oracle's direct sum over all j, for the mixed form's pairwise kernel, its
+    far field, and the plain form's kernel with the hardware rsq the
+    reference uses.  Bounds: max 1e-5 and median 1e-7 (mixed, far field),
+    max 1e-5 and median 1e-6 (plain); potentials to 2e-6."""
+    n = 1 << 20
+    print(f"full-size gravity per particle cfg={cfg} n={n}", flush=True)
+    width = 128
+    windows = np.linspace(0, n - width, 16).astype(np.int64)
+    e = cs.make_engine("mixed", cfg, n, 12345, ph_sort=True, i_order=0)
+    e.run_pass(cs.GRAVITY)
+    g_pair = e.get("grav")[:n].astype(np.float64)
+    e.set_scalar("grav_far", 1)
+    e.run_pass(cs.GRAVITY)
+    g_far = e.get("grav")[:n].astype(np.float64)
+    pos64, vel64 = e.get("pos64")[:n].copy(), e.get("vel64")[:n].copy()
+    p = e.params.copy()
+    e.close()
+    p.n_threads = 16
+    idx, ref = sampled_fp64_gravity(orc, p, pos64, vel64, windows, width)
+    print("  oracle (mixed state) done", flush=True)
+    for name, g, med_bound in (("pairwise mixed", g_pair, 1e-7), ("far field", g_far, 1e-7)):
+        err, perr = gravity_errors(g[idx], ref)
+        print(f"  {name}: median {np.median(err):.2e} max {err.max():.2e} potential max {perr.max():.2e}", flush=True)
+        assert err.max() < 1e-5 and np.median(err) < med_bound, (name, err.max(), np.median(err))
+        assert perr.max() < 2e-6, (name, perr.max())
+
+    pl = cs.make_engine("plain", cfg, n, 12345, ph_sort=True, i_order=0)
+    pl.set_scalar("exact_rsqrt", 0)
+    pl.run_pass(cs.GRAVITY)
+    g_plain = pl.get("grav")[:n].astype(np.float64)
+    ppos, pvel = pl.get("pos")[:n].astype(np.float64), pl.get("vel")[:n].astype(np.float64)
+    pp = pl.params.copy()
+    pl.close()
+    pp.n_threads = 16
+    idx, ref = sampled_fp64_gravity(orc, pp, ppos, pvel, windows, width)
+    err, perr = gravity_errors(g_plain[idx], ref)
+    print(f"  plain (hardware rsq): median {np.median(err):.2e} max {err.max():.2e} "
+          f"potential max {perr.max():.2e}", flush=True)
+    assert err.max() < 1e-5 and np.median(err) < 1e-6, (err.max(), np.median(err))
+    assert perr.max() < 2e-6, perr.max()
+
+
+@pytest.mark.parametrize("cfg", [3, 5])
+def test_mixed_sph_passes_against_fp64_oracle_at_full_size(cs, orc, cfg):
+    """One SPH force evaluation of the mixed form at N = 2^20 against the FP64
+    oracle run on the mixed form's own FP64 state and ranges (tree, lists,
+    density, EOS, hydro, reaction in double), over the particles whose lists
+    agree: rho and P to 2e-6; hydro accelerations, relative to
+    max(|a|, 1e-3 rms), median 1e-6, 99.9th percentile 1e-5, max 1e-4 (the
+    pressure forces of a star in hydrostatic balance cancel to a net force
+    well below their pair terms, which the FP32 pair arithmetic rounds);
+    du/dt the same where the state has one (the turbulent sphere: each
+    polytrope moves rigidly at t = 0, so there du/dt vanishes to rounding)."""
+    n = 1 << 20
+    print(f"full-size mixed SPH passes cfg={cfg} n={n}", flush=True)
+    e = cs.make_engine("mixed", cfg, n, 12345, ph_sort=True, i_order=0)
+    e.run_pass(cs.SPH_ACC)
+    p = e.params.copy()
+    p.n_threads = 16
+    o = orc.Oracle(64, p)
+    P = e.get("pos").astype(np.float64)
+    P[:n, :3] = e.get("pos64")[:n, :3]
+    V = e.get("vel64").copy()
+    V[n:] = e.get("vel")[n:]
+    H = e.get("hydro").astype(np.float64)
+    H[:n, 2] = e.get("u64")[:n]
+    o.set("pos", P)
+    o.set("vel", V)
+    o.set("hydro", H)
+    o.set("n_ngb", e.get("n_ngb"))
+    for pid in (orc.TREE, orc.OFFSETS, orc.FIND, orc.DENSITY, orc.EOS, orc.HYDRO, orc.REACTION):
+        o.run_pass(pid)
+    print("  oracle done", flush=True)
+    same = e.get("ngb_count")[:n] == o.get("ngb_count")[:n]
+    assert same.mean() > 0.999, same.mean()
+    he, ho = e.get("hydro")[:n].astype(np.float64), o.get("hydro")[:n]
+    rho_err = (np.abs(he[:, 0] - ho[:, 0]) / ho[:, 0])[same]
+    p_err = (np.abs(he[:, 1] - ho[:, 1]) / ho[:, 1])[same]
+    off, cnt, lst = o.get("ngb_offset"), o.get("ngb_count"), o.get("ngb_list")
+    ok = same.copy()
+    for i in np.nonzero(~same)[0]:
+        ok[lst[off[i]:off[i] + cnt[i]]] = False
+    a, b = e.get("bdots")[:n].astype(np.float64)[ok], o.get("bdots")[:n][ok]
+    floor = 1e-3 * np.sqrt((b[:, :3] ** 2).sum(1).mean())
+    a_err = np.linalg.norm(a[:, :3] - b[:, :3], axis=1) / np.maximum(np.linalg.norm(b[:, :3], axis=1), floor)
+    print(f"  lists equal {same.mean():.6f}; rho max {rho_err.max():.2e}, P max {p_err.max():.2e}, "
+          f"a median {np.median(a_err):.2e} 99.9% {np.quantile(a_err, 0.999):.2e} max {a_err.max():.2e}", flush=True)
+    assert rho_err.max() < 2e-6 and p_err.max() < 2e-6
+    assert np.median(a_err) < 1e-6 and np.quantile(a_err, 0.999) < 1e-5 and a_err.max() < 1e-4
+    if cfg == 5:
+        u_err = np.abs(a[:, 3] - b[:, 3]) / np.maximum(np.abs(b[:, 3]), 1e-3 * np.sqrt((b[:, 3] ** 2).mean()))
+        print(f"  du/dt median {np.median(u_err):.2e} 99.9% {np.quantile(u_err, 0.999):.2e} max {u_err.max():.2e}",
+              flush=True)
+        assert np.median(u_err) < 1e-6 and np.quantile(u_err, 0.999) < 1e-5 and u_err.max() < 1e-4
