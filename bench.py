@@ -36,10 +36,11 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# MI355X's published FP32 vector peak: 256 CUs x 4 SIMDs x 2.4 GHz x 64 flops
+# per SIMD per cycle, i.e. a wave64 FMA every two cycles.  This build emits no
+# packed FP32 (v_pk_*); the measured single-lane FMA loop (k_fma_peak) reaches
+# 0.8 of this figure, so it is the unpacked rate, not a packed one.
 PEAK_FP32_TFLOPS = 157.3
-# the same peak without packed FP32 (v_pk_fma_f32), which this build does not
-# emit: one FP32 FMA per lane per instruction
-PEAK_FP32_UNPACKED_TFLOPS = 78.6
 FLOP_PER_PAIR = 20
 WORKLOAD = "two n=1.5 polytropes (R=1, M=0.5 each), parabolic orbit q=0.5 from separation 5; " \
            "gas+radiation EOS mu=0.6 a=1e-3; 64+-8 neighbours; Plummer eps=0.002"
@@ -487,8 +488,6 @@ def main():
         "roofline": {"bound": "valu", "kernel": "k_gravity_mixed_plummer",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profile("k_gravity"),
-                     "peak_unpacked": PEAK_FP32_UNPACKED_TFLOPS,
-                     "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
                      "frac_of_measured_fma": achieved / peaks["fp32_fma_TFLOPs"],
                      "frac_of_pair_body": achieved / peaks["gravity_pair_body_TFLOPs"],
                      "per_launch": f"{FLOP_PER_PAIR} flop x {nd}^2 pairs",

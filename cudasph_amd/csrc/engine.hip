@@ -247,7 +247,6 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_split_only") return grav_split_only_;
   if (name == "grav_unit_mass") return grav_unit_mass_;
   if (name == "grav_expand") return grav_expand_;
-  if (name == "grav_mfma") return grav_mfma_;
   if (name == "solve_cache") return solve_cache_on_;
   if (name == "adj_free_ranges") return adj_free_ranges_;
   if (name == "adj_implicit_h") return adj_implicit_h_;
@@ -284,8 +283,6 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     grav_unit_mass_ = (int)v;
   } else if (name == "grav_expand") {
     grav_expand_ = (int)v;
-  } else if (name == "grav_mfma") {
-    grav_mfma_ = (int)v;
   } else if (name == "solve_cache") {
     solve_cache_on_ = (int)v;
   } else if (name == "adj_free_ranges") {
@@ -627,7 +624,6 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
                            n_, p_.grav_mode,
                            p_.grav_eps,
                            grav_split_only_ | (grav_unit_mass_ ? 0 : 2) | (grav_expand_ ? 0 : 4) |
-                               (grav_mfma_ ? 128 : 0) |
                                (grav_far_ ? 16 : 0) | (grav_far_ && grav_far_half_ ? 64 : 0),
                            gs);
       return;

@@ -528,7 +528,7 @@ __device__ __forceinline__ void far_field(const float* __restrict__ M, float yx,
 // registers without spilling (at 6 it spills 15 registers and runs 4-8 %
 // slower).  The pairwise instantiation is left unbounded (63 registers, 8
 // wavefronts).
-template <int JT, bool FAR, bool MF>
+template <int JT, bool FAR>
 __global__ __launch_bounds__(kGravBlock) __attribute__((amdgpu_waves_per_eu(FAR ? 5 : 1)))
 void k_gravity_mixed_plummer(const F4* __restrict__ hi,
                                                                        const F4* __restrict__ lo,
@@ -553,17 +553,9 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
   const bool far_on = FAR && (split_only & 1) == 0;
   // split_only bit 6: 32-record halves of a sub-tile not taken whole
   const bool half_on = far_on && (split_only & 64) != 0 && halves != nullptr;
-  // MF (split_only bit 7): the expanded body's r^2 products and sums on the
-  // matrix cores, a separate instantiation (its registers would lower the
-  // VALU body's occupancy)
-  constexpr bool use_mfma = MF;
   split_only &= 1;
   constexpr int kSub = JT / kTile;   // sub-tiles of kTile records, each with its own origin
   __shared__ F4 s_hi[JT], s_lo[JT], s_rel[JT];
-  // the staged records' (x, y, z, 1) in groups of four, transposed: float
-  // [4 g + c][v] is component c of record 4 g + v (the matrix operand each
-  // lane supplies for the sums over a group of four records)
-  __shared__ __align__(16) float s_relT[MF ? JT * 4 : 4];
   __shared__ F4 s_c[kSub];
   __shared__ float s_r2[kSub];
   __shared__ __align__(16) float s_mom[kSub][FAR ? kMomWords : 1];
@@ -650,13 +642,6 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
           s_hi[q] = h;
           s_lo[q] = l;
           s_rel[q] = d;
-          if constexpr (MF) {
-            const int g = q >> 2, v = q & 3;
-            s_relT[(4 * g + 0) * 4 + v] = d.x;
-            s_relT[(4 * g + 1) * 4 + v] = d.y;
-            s_relT[(4 * g + 2) * 4 + v] = d.z;
-            s_relT[(4 * g + 3) * 4 + v] = 1.f;
-          }
         }
         for (int o = 32; o > 0; o >>= 1) r2 = fmaxf(r2, __shfl_xor(r2, o));
         if (lane == 0) {
@@ -762,51 +747,7 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
       auto run = [&](int base, auto len_c) {
         constexpr int len = decltype(len_c)::value;
         if (uniform) {
-          if (tile_form && expand && use_mfma) {
-            // The expanded equal-mass body with its products on the matrix
-            // cores.  v_mfma_f32_4x4x1_16b_f32 adds, in each of 16 blocks of
-            // four lanes, a 4x1 by 1x4 outer product to a 4x4 tile:
-            //   D[v](lane l) = C[v](lane l) + A(lane 4 (l / 4) + v) * B(lane l),
-            // so every lane keeps its own i-particle.  For four records
-            // j0..j0+3 at a time:
-            //   r^2: A(lane L) = a component of record j0 + L % 4 and B the
-            //        lane's coefficient (-2 d_i, or 1 for |d_j|^2), C = c_i;
-            //        D[v] is then r^2 of (i, j0 + v), four products deep;
-            //   sums: A(lane L) = component L % 4 of (d_j, 1) of record j0 + v
-            //        and B = s3 of (i, j0 + v): D accumulates sum_j s3 d_j and
-            //        sum_j s3 over the sub-tile.
-            // The VALU keeps the rsq, the cube and the potential.
-            typedef float v4f __attribute__((ext_vector_type(4)));
-            const float ci = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
-            const float bx = -2.f * dx, by = -2.f * dy, bz = -2.f * dz;
-            const int q4 = lane & 3;
-            const float4* relT = reinterpret_cast<const float4*>(s_relT);
-            v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-            for (int g = 0; g < len / 4; ++g) {
-              const F4 rec = s_rel[base + 4 * g + q4];
-              const float4 wt = relT[(base / 4 + g) * 4 + q4];
-              v4f d2 = {ci, ci, ci, ci};
-              d2 = __builtin_amdgcn_mfma_f32_4x4x1f32(rec.w, 1.f, d2, 0, 0, 0);
-              d2 = __builtin_amdgcn_mfma_f32_4x4x1f32(rec.z, bz, d2, 0, 0, 0);
-              d2 = __builtin_amdgcn_mfma_f32_4x4x1f32(rec.y, by, d2, 0, 0, 0);
-              d2 = __builtin_amdgcn_mfma_f32_4x4x1f32(rec.x, bx, d2, 0, 0, 0);
-              float s3[4];
-#pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                const float inv = __builtin_amdgcn_rsqf(d2[v]);
-                s3[v] = inv * (inv * inv);
-                a[3].sub(inv);
-              }
-              acc = __builtin_amdgcn_mfma_f32_4x4x1f32(wt.x, s3[0], acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_4x4x1f32(wt.y, s3[1], acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_4x4x1f32(wt.z, s3[2], acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_4x4x1f32(wt.w, s3[3], acc, 0, 0, 0);
-            }
-            a[0].part += __builtin_fmaf(-dx, acc[3], acc[0]);
-            a[1].part += __builtin_fmaf(-dy, acc[3], acc[1]);
-            a[2].part += __builtin_fmaf(-dz, acc[3], acc[2]);
-          } else if (tile_form && expand) {
+          if (tile_form && expand) {
             // expanded equal-mass body: r^2 + eps^2 = (|d_i|^2 + eps^2 + |d_j|^2)
             // - 2 d_i.d_j and sum_j s3 (d_j - d_i) = M - d_i S over the
             // records, 11 VALU + rsq per pair.  With |d_j| <= |d_i| / 3 the
@@ -1058,26 +999,12 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
       fq.q2[L] = q * q;
     }
     const dim3 ggrid(ceil_div(n_i, kGravBlock) * kGravParts);
-    const bool mf = (split_only & 128) != 0;
-    if ((split_only & 16) && moments) {
-      if (mf)
-        hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, true, true>), ggrid, block, 0, s, hi, lo, part_hi,
-                           part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range, moments, fq, cells, halves,
-                           n_real);
-      else
-        hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, true, false>), ggrid, block, 0, s, hi, lo, part_hi,
-                           part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range, moments, fq, cells, halves,
-                           n_real);
-    } else {
-      if (mf)
-        hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, false, true>), ggrid, block, 0, s, hi, lo, part_hi,
-                           part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range, nullptr, fq, nullptr,
-                           nullptr, n_real);
-      else
-        hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, false, false>), ggrid, block, 0, s, hi, lo,
-                           part_hi, part_lo, n_i, n_j, chunk, eps * eps, split_only, mass_range, nullptr, fq,
-                           nullptr, nullptr, n_real);
-    }
+    if ((split_only & 16) && moments)
+      hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, true>), ggrid, block, 0, s, hi, lo, part_hi, part_lo,
+                         n_i, n_j, chunk, eps * eps, split_only, mass_range, moments, fq, cells, halves, n_real);
+    else
+      hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, false>), ggrid, block, 0, s, hi, lo, part_hi, part_lo,
+                         n_i, n_j, chunk, eps * eps, split_only, mass_range, nullptr, fq, nullptr, nullptr, n_real);
     CS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gravity_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part_hi, part_lo, out,
                        n_i, mass_range, split_only);

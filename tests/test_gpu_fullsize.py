@@ -225,7 +225,10 @@ def test_mixed_sph_passes_against_fp64_oracle_at_full_size(cs, orc, cfg):
     pressure forces of a star in hydrostatic balance cancel to a net force
     well below their pair terms, which the FP32 pair arithmetic rounds);
     du/dt the same where the state has one (the turbulent sphere: each
-    polytrope moves rigidly at t = 0, so there du/dt vanishes to rounding)."""
+    polytrope moves rigidly at t = 0, so there du/dt vanishes to rounding),
+    relative to the larger of |du/dt| and its gross rate (P/rho)(|div v| +
+    |curl v|): where compression and shear cancel, du/dt is a small
+    difference of terms of that size."""
     n = 1 << 20
     print(f"full-size mixed SPH passes cfg={cfg} n={n}", flush=True)
     e = cs.make_engine("mixed", cfg, n, 12345, ph_sort=True, i_order=0)
@@ -263,7 +266,10 @@ def test_mixed_sph_passes_against_fp64_oracle_at_full_size(cs, orc, cfg):
     assert rho_err.max() < 2e-6 and p_err.max() < 2e-6
     assert np.median(a_err) < 1e-6 and np.quantile(a_err, 0.999) < 1e-5 and a_err.max() < 1e-4
     if cfg == 5:
-        u_err = np.abs(a[:, 3] - b[:, 3]) / np.maximum(np.abs(b[:, 3]), 1e-3 * np.sqrt((b[:, 3] ** 2).mean()))
+        dots, hyd = o.get("dots")[:n][ok], ho[ok]
+        gross = hyd[:, 1] / hyd[:, 0] * (np.abs(dots[:, 0]) + dots[:, 1])
+        u_err = np.abs(a[:, 3] - b[:, 3]) / np.maximum(np.maximum(np.abs(b[:, 3]), gross),
+                                                        1e-3 * np.sqrt((b[:, 3] ** 2).mean()))
         print(f"  du/dt median {np.median(u_err):.2e} 99.9% {np.quantile(u_err, 0.999):.2e} max {u_err.max():.2e}",
               flush=True)
         assert np.median(u_err) < 1e-6 and np.quantile(u_err, 0.999) < 1e-5 and u_err.max() < 1e-4
