@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -156,9 +157,60 @@ void write_state_from(EngineBase* e, FILE* f, int n, int nd) {
   }
 }
 
+// Scratch of sph_radix_sort_noalloc: the digit-count and offset tables and
+// the scan partials of launch_radix_sort for up to SPH_NOALLOC_SORT_MAX
+// records (4096 records per tile, 256 digits; radix_scratch_ints).  Each GPU
+// the library runs on holds its own copy until the process ends.  Host code
+// never uses this name as an address: sort_scratch_ptr() looks the device
+// address up once per device.
+constexpr long kNoAllocSortMax = SPH_NOALLOC_SORT_MAX;
+constexpr long kNoAllocTable = 256L * ((kNoAllocSortMax + 4095) / 4096);
+constexpr long kNoAllocScratchInts = 2 * kNoAllocTable + 2 + (kNoAllocTable + 2047) / 2048 + 1;
+static __device__ __align__(256) int g_sort_scratch[kNoAllocScratchInts];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's address of g_sort_scratch (hipGetSymbolAddress on the
+// first call on each device, with that device current)
+int* sort_scratch_ptr() {
+  int dev = 0;
+  CS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("sph_radix_sort_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    if (radix_scratch_ints(kNoAllocSortMax) > kNoAllocScratchInts)
+      throw std::logic_error("sph_radix_sort_noalloc: scratch array smaller than the sort needs");
+    void* p = nullptr;
+    CS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(g_sort_scratch)));
+    g_scratch_ptr[dev] = static_cast<int*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 extern "C" {
+
+// RadixSort (radixsort.cu:50-77) without any allocation: the scratch comes
+// from the library's fixed device array
+int sph_radix_sort_noalloc(sph_kv* pData0, sph_kv* pData1, unsigned int elements, unsigned int bits,
+                           void* stream) {
+  return guard([&] {
+    if (elements > (unsigned)kNoAllocSortMax)
+      throw std::invalid_argument("sph_radix_sort_noalloc: " + std::to_string(elements) +
+                                  " elements exceed the maximum of " + std::to_string(kNoAllocSortMax));
+    if (bits > 32) throw std::invalid_argument("sph_radix_sort_noalloc: bits must be at most 32");
+    if (elements > 1 && bits > 0 && (!pData0 || !pData1))
+      throw std::invalid_argument("sph_radix_sort_noalloc: null data pointer");
+    int* scratch = sort_scratch_ptr();
+    launch_radix_sort(reinterpret_cast<KV*>(pData0), reinterpret_cast<KV*>(pData1), (long)elements, (int)bits,
+                      scratch, static_cast<hipStream_t>(stream));
+    return 0;
+  }, -1);
+}
+
 
 const char* sph_last_error(void) { return g_last_error.c_str(); }
 

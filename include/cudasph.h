@@ -87,6 +87,21 @@ int cuda_get_device(void);
  * same size.  Sorted output ends in pData0. */
 void RadixSort(sph_kv* pData0, sph_kv* pData1, unsigned int elements, unsigned int bits);
 
+/* No counterpart in the reference: RadixSort for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The same stable LSD sort of the low
+ * `bits` bits of key, result in pData0, pData1 scratch of the same size; the
+ * digit tables live in a fixed device array of this library (about 8.4 MB
+ * per GPU, held until the process ends) instead of a hipMalloc per call, so
+ * at most SPH_NOALLOC_SORT_MAX elements.  Enqueued on `stream` (a
+ * hipStream_t; NULL is the null stream) and not waited for.  Every call on a
+ * GPU uses that GPU's one array: calls that overlap on one GPU race.
+ * Returns 0, or -1 with sph_last_error() (more than SPH_NOALLOC_SORT_MAX
+ * elements, bits above 32, a null pointer, a HIP error). */
+#define SPH_NOALLOC_SORT_MAX (1 << 24)
+int sph_radix_sort_noalloc(sph_kv* pData0, sph_kv* pData1, unsigned int elements, unsigned int bits,
+                           void* stream);
+
 /* kd_tree.cpp:97-128 with the packing of sph_accelerations.cpp:21-33 (host
  * code, no GPU): the reference's left-balanced kd-tree of pos[0..n) (x,y,z
  * used), built with the reference's quicksort so ties resolve identically,
