@@ -152,7 +152,7 @@ float host_solve_range(int n_bodies, int n_nodes, float rel_err, sph_kv* bodies_
     CS_HIP(hipMemcpy(a.tree, kd_tree, ((size_t)n_bodies + 1) * sizeof(F4), hipMemcpyDeviceToDevice));
     CS_HIP(hipMemset(a.tree + n_bodies + 1, 0, sizeof(F4)));
     launch_solve_range<float>((F4*)bodies_pos, n_ngb, (const KV*)bodies_hash, a.tree, n_bodies,
-                              walk(a, n_nodes, rel_err), nullptr, 0, nullptr);
+                              walk(a, n_nodes, rel_err), nullptr);
     CS_HIP(hipDeviceSynchronize());
     a.pos = (F4*)bodies_pos;
     a.vel = (F4*)bodies_vel;

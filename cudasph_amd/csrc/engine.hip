@@ -180,7 +180,10 @@ long Engine<T>::set(const std::string& name, const void* host, long nb) {
   if (name == "hash") return put(hash_, hash_.bytes());
   if (name == "n_ngb") return put(n_ngb_, n_ngb_.bytes());
   if (name == "ngb_count") return put(ngb_count_, ngb_count_.bytes());
-  if (name == "ngb_offset") return put(ngb_offset_, ngb_offset_.bytes());
+  if (name == "ngb_offset") {
+    tot_stale_ = true;
+    return put(ngb_offset_, ngb_offset_.bytes());
+  }
   if (name == "ngb_list") return put(list_, list_.bytes());
   if (name == "ngb_acc") return put(acc_, acc_.bytes());
   if (mixed()) {
@@ -247,7 +250,6 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_split_only") return grav_split_only_;
   if (name == "grav_unit_mass") return grav_unit_mass_;
   if (name == "grav_expand") return grav_expand_;
-  if (name == "solve_cache") return solve_cache_on_;
   if (name == "adj_free_ranges") return adj_free_ranges_;
   if (name == "adj_implicit_h") return adj_implicit_h_;
   if (name == "grav_far") return grav_far_;
@@ -283,8 +285,6 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     grav_unit_mass_ = (int)v;
   } else if (name == "grav_expand") {
     grav_expand_ = (int)v;
-  } else if (name == "solve_cache") {
-    solve_cache_on_ = (int)v;
   } else if (name == "adj_free_ranges") {
     adj_free_ranges_ = (int)v;
   } else if (name == "adj_implicit_h") {
@@ -422,9 +422,7 @@ void Engine<T>::solve() {
   }
   {
     TimerScope ts(timers_, "solve_range", s_);
-    if (solve_cache_on_ && solve_cache_.n == 0) solve_cache_.alloc((size_t)nd_ * kSolveCacheCap);
-    launch_solve_range<T>(pos_, n_ngb_, hash_, tree_, nd_, walk_params(), solve_cache_on_ ? solve_cache_.p : nullptr,
-                          kSolveCacheCap, s_);
+    launch_solve_range<T>(pos_, n_ngb_, hash_, tree_, nd_, walk_params(), s_);
   }
   if (adj_mode_ == kAdjRecord) {
     launch_get_range<T>(pos_, adj_range_.p + rec, nd_, s_);
@@ -452,6 +450,7 @@ long Engine<T>::check_total() {
   if (tot < 0 || tot > cap_)
     throw std::runtime_error("neighbour list overflow: ngb_tot exceeds 100 n_dev (sph_accelerations.cpp:66-72)");
   ngb_tot_ = tot;
+  tot_stale_ = false;
   return tot;
 }
 
@@ -461,7 +460,7 @@ long Engine<T>::check_total() {
 template <class T>
 void Engine<T>::find() {
   const WalkParams wp = walk_params();
-  check_total();
+  if (tot_stale_) check_total();   // offsets set from outside since the last scan
   for (int attempt = 0; attempt < 2; ++attempt) {
     {
       TimerScope ts(timers_, "find_ngb", s_);
@@ -532,7 +531,7 @@ void Engine<T>::hydro() {
 // (the reference's serial order), and each j sums its own reactions
 template <class T>
 void Engine<T>::build_transpose() {
-  check_total();
+  if (tot_stale_) check_total();
   launch_reaction_transpose(ngb_offset_, ngb_count_, list_, nd_, owner_, tr_start_, tr_end_, pairs_, pairs_alt_,
                             scan_tmp_, s_);
 }

@@ -215,13 +215,8 @@ class Engine final : public EngineBase {
   // 0: on the host, bit-identical to the reference's quicksort build
   int tree_gpu_ = 0;
   DBuf<int> tree_work_;   // allocated on first use of the device build
-  // h-solve candidate cache: kSolveCacheCap distances per thread, allocated on
-  // first use (1: on, the default; 0: every evaluation walks the tree -- the
-  // same bits either way)
-  static constexpr int kSolveCacheCap = 192;
-  int solve_cache_on_ = 1;
-  DBuf<T> solve_cache_;
   long ngb_tot_ = 0;
+  bool tot_stale_ = true;   // ngb_tot_ not yet read for the current offsets (a host round trip)
   int overflow_reruns_ = 0;
   int phantom_reads_ = 0, phantom_ngb_ = 0;   // walks of the last find_ngb that read node n+1 / took it
 };

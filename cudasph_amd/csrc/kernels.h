@@ -17,11 +17,9 @@ struct WalkParams {
 struct EosConst;  // sph_math.h
 
 // ---- sph_passes.hip ----
-// cache: n_dev * cap reals of per-thread candidate distances (nullptr: each
-// evaluation walks the tree); the results are the same bits either way
 template <class T>
 void launch_solve_range(V4<T>* pos, int* n_ngb, const KV* hash, const V4<T>* tree, int n_dev,
-                        const WalkParams& wp, T* cache, int cap, hipStream_t s);
+                        const WalkParams& wp, hipStream_t s);
 template <class T>
 void launch_find_ngb(const V4<T>* pos, const V4<T>* tree, KV* hash, const int* offset, const int* cap_count,
                      int* count_out, int* list, int* flags, int n_dev, const WalkParams& wp, int clamp,

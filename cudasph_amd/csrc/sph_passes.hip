@@ -67,58 +67,18 @@ struct NgbEval {
   T f, step, count;
 };
 
-// Candidate cache of one lane's h-solve.  The walk visits tree nodes in a
-// fixed order (a node, then its right subtree, then its left), and a larger
-// query radius only prunes less (fl(q - R) and fl(q + R) are monotone in R),
-// so the nodes a walk at radius r <= R visits, and their distances, are
-// exactly the walk at R's, in the same order, filtered by ds2 < r^2.  The
-// first evaluation therefore walks at R = 9/8 of its own radius (the bracket
-// moves by at most a tenth of the range per step) and stores every visited
-// distance; each later evaluation with r <= R sums the stored distances below
-// r^2 in order -- the same terms in the same order as its own walk, so the
-// same bits -- and only a radius beyond R (or more candidates than the cache
-// holds) walks the tree again.
-template <class T>
-struct NgbCache {
-  T* buf = nullptr;   // this lane's `cap` slots; nullptr: walk every time
-  int cap = 0;
-  int cnt = 0;
-  T rc = T(-1);       // radius of the cached walk; negative: nothing cached
-};
-
 // smoothed count minus target, Newton step and integer count at range q.w
 // (dev_solve_range.cu:17-89)
 template <class T>
 __device__ __forceinline__ NgbEval<T> eval_ngb(const V4<T>& q, const V4<T>* __restrict__ tree,
-                                               const WalkParams& wp, int* stk, NgbCache<T>& cache) {
+                                               const WalkParams& wp, int* stk) {
   T f = T(0), df = T(0), cnt = T(0);
-  auto add = [&](T ds2) {
+  kd_walk<T>(q, tree, wp.n_nodes, wp.fix_phantom, stk, [&](T ds2, int) {
     const T u = T(2) * dsqrt(ds2) / q.w;
     f += kern_ig<T>(u);
     df -= kern_digdh<T>(u);
     cnt += T(1);
-  };
-  if (cache.buf && q.w <= cache.rc) {
-    const T r2 = q.w * q.w;
-    for (int k = 0; k < cache.cnt; ++k) {
-      const T ds2 = cache.buf[k];
-      if (ds2 < r2) add(ds2);
-    }
-  } else if (cache.buf) {
-    V4<T> qc = q;
-    qc.w = q.w * T(1.125);
-    const T r2 = q.w * q.w;
-    int m = 0;
-    kd_walk<T>(qc, tree, wp.n_nodes, wp.fix_phantom, stk, [&](T ds2, int) {
-      if (m < cache.cap) cache.buf[m] = ds2;
-      ++m;
-      if (ds2 < r2) add(ds2);
-    });
-    cache.cnt = m;
-    cache.rc = m <= cache.cap ? qc.w : T(-1);
-  } else {
-    kd_walk<T>(q, tree, wp.n_nodes, wp.fix_phantom, stk, [&](T ds2, int) { add(ds2); });
-  }
+  });
   f -= T(wp.desired_ngb);
   df = T((double)df * (1.0 / (double)q.w));
   if (df != T(0)) df = f / df;
@@ -135,9 +95,9 @@ __device__ __forceinline__ T clamp_step(T dx, T range) {   // f_scale = 0.1, dev
 // bracket [lo, hi] around the target (dev_solve_range.cu:93-135); z = count
 template <class T>
 __device__ __forceinline__ V4<T> bracket(V4<T> q, const V4<T>* __restrict__ tree, const WalkParams& wp,
-                                         int* stk, NgbCache<T>& cache) {
+                                         int* stk) {
   const T rel = T(wp.rel_err);
-  NgbEval<T> f = eval_ngb<T>(q, tree, wp, stk, cache);
+  NgbEval<T> f = eval_ngb<T>(q, tree, wp, stk);
   T dx = f.step;
   if (dabs(dx) < rel * q.w)
     return {T((double)q.w - 0.5 * (double)dabs(dx)), T((double)q.w + 0.5 * (double)dabs(dx)), f.count, T(0)};
@@ -148,7 +108,7 @@ __device__ __forceinline__ V4<T> bracket(V4<T> q, const V4<T>* __restrict__ tree
     x_prev = q.w;
     f_prev = f.f;
     q.w -= dx;
-    f = eval_ngb<T>(q, tree, wp, stk, cache);
+    f = eval_ngb<T>(q, tree, wp, stk);
     dx = f.step;
     if (dabs(dx) < rel * q.w)
       return {T((double)q.w - 0.5 * (double)dabs(dx)), T((double)q.w + 0.5 * (double)dabs(dx)), f.count, T(0)};
@@ -161,9 +121,9 @@ __device__ __forceinline__ V4<T> bracket(V4<T> q, const V4<T>* __restrict__ tree
 // safeguarded Newton inside the bracket (dev_solve_range.cu:137-172)
 template <class T>
 __device__ __forceinline__ void solve_range(V4<T> q, const V4<T>* __restrict__ tree, const WalkParams& wp,
-                                            int* stk, NgbCache<T>& cache, T& range, T& count) {
+                                            int* stk, T& range, T& count) {
   const T rel = T(wp.rel_err);
-  V4<T> br = bracket<T>(q, tree, wp, stk, cache);
+  V4<T> br = bracket<T>(q, tree, wp, stk);
   T lo = br.x, hi = br.y;
   count = br.z;
   q.w = T(0.5 * (double)(lo + hi));
@@ -177,7 +137,7 @@ __device__ __forceinline__ void solve_range(V4<T> q, const V4<T>* __restrict__ t
         dx = T(0.5 * (double)(hi - lo));
         q.w = T(0.5 * (double)(lo + hi));
       }
-      const NgbEval<T> f = eval_ngb<T>(q, tree, wp, stk, cache);
+      const NgbEval<T> f = eval_ngb<T>(q, tree, wp, stk);
       count = f.count;
       if (f.f < T(0)) lo = q.w;
       else hi = q.w;
@@ -188,25 +148,19 @@ __device__ __forceinline__ void solve_range(V4<T> q, const V4<T>* __restrict__ t
   range = q.w;
 }
 
-// dev_solve_range.cu:174-189; with `cache` given, thread t keeps its
-// candidate distances in cache[t * cap, (t + 1) * cap)
+// dev_solve_range.cu:174-189
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_solve_range(V4<T>* __restrict__ pos, int* __restrict__ n_ngb,
                                                          const KV* __restrict__ hash,
                                                          const V4<T>* __restrict__ tree, int n_dev,
-                                                         WalkParams wp, T* __restrict__ cache, int cap) {
+                                                         WalkParams wp) {
   extern __shared__ int s_stack[];   // walk_stack_depth(n_nodes) entries per lane
   const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
   const V4<T> q = pos[i];
   T range = T(0), count = T(0);
-  NgbCache<T> c;
-  if (cache) {
-    c.buf = cache + (size_t)t * cap;
-    c.cap = cap;
-  }
-  if (q.w > T(0)) solve_range<T>(q, tree, wp, s_stack + threadIdx.x, c, range, count);
+  if (q.w > T(0)) solve_range<T>(q, tree, wp, s_stack + threadIdx.x, range, count);
   pos[i].w = range;
   n_ngb[i] = (int)count;
 }
@@ -599,10 +553,10 @@ __global__ __launch_bounds__(kBlock) void k_hydro_mixed(const F4* __restrict__ p
 // launchers
 template <class T>
 void launch_solve_range(V4<T>* pos, int* n_ngb, const KV* hash, const V4<T>* tree, int n_dev,
-                        const WalkParams& wp, T* cache, int cap, hipStream_t s) {
+                        const WalkParams& wp, hipStream_t s) {
   const size_t lds = (size_t)walk_stack_depth(wp.n_nodes) * kBlock * sizeof(int);
   hipLaunchKernelGGL(k_solve_range<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), lds, s, pos, n_ngb, hash,
-                     tree, n_dev, wp, cache, cache ? cap : 0);
+                     tree, n_dev, wp);
   CS_CHECK_LAUNCH();
 }
 
@@ -661,8 +615,8 @@ void launch_hydro(const V4<T>* pos, const V4<T>* vel, const V4<T>* hydro, const 
 }
 
 #define CS_INST(T)                                                                                        \
-  template void launch_solve_range<T>(V4<T>*, int*, const KV*, const V4<T>*, int, const WalkParams&, T*, \
-                                      int, hipStream_t);                                                  \
+  template void launch_solve_range<T>(V4<T>*, int*, const KV*, const V4<T>*, int, const WalkParams&,      \
+                                      hipStream_t);                                                       \
   template void launch_find_ngb<T>(const V4<T>*, const V4<T>*, KV*, const int*, const int*, int*, int*,   \
                                    int*, int, const WalkParams&, int, hipStream_t);                       \
   template void launch_eos<T>(V4<T>*, int, const EosConst&, hipStream_t);

@@ -279,6 +279,10 @@ void   sph_engine_reset_timers(void* e);
  *                        counts afresh at every evaluation (timesteps still
  *                        recorded): the map whose derivative the frozen-range
  *                        gradient approximates.
+ * With the engine scalar adj_implicit_h = 1, sph_adjoint_backward adds each
+ * evaluation's h-solve derivative (dr/dx through the smoothed-count equation)
+ * and returns the gradient of that re-solved map instead of the frozen one
+ * (meaningful where the h-solve is converged tightly, e.g. rel_err 1e-10).
  * All return 0, or -1 with sph_last_error() set. */
 int    sph_adjoint_forward(void* e, int steps, int ckpt_every, double* loss);
 int    sph_adjoint_backward(void* e, double* grad_x, double* grad_v, double* grad_u);

@@ -414,22 +414,3 @@ def test_mixed_state_file_keeps_fp64_state(cs, tmp_path):
     e.close()
     e2.close()
 
-
-@pytest.mark.parametrize("form,cfg,n", [("plain", 2, 65536), ("plain", 3, 65536), ("fp64", 2, 16384),
-                                        ("mixed", 3, 65536)])
-def test_solve_cache_changes_nothing(cs, form, cfg, n):
-    """The h-solve's candidate cache (one walk at 9/8 of the starting range,
-    later evaluations filter its distances in walk order) gives the same
-    ranges and counts, bit for bit, as walking the tree for every
-    evaluation, and the whole step after it the same bits."""
-    print(f"solve cache form={form} cfg={cfg} n={n}", flush=True)
-    out = []
-    for on in (0, 1):
-        e = cs.make_engine(form, cfg, n, 17, ph_sort=True)
-        e.set_scalar("solve_cache", on)
-        e.init()
-        e.step()
-        out.append({k: e.get(k)[:n].copy() for k in ("pos", "n_ngb", "bdots", "hydro")})
-        e.close()
-    for k in out[0]:
-        assert np.array_equal(out[0][k], out[1][k]), k
