@@ -273,3 +273,46 @@ def test_mixed_sph_passes_against_fp64_oracle_at_full_size(cs, orc, cfg):
         print(f"  du/dt median {np.median(u_err):.2e} 99.9% {np.quantile(u_err, 0.999):.2e} max {u_err.max():.2e}",
               flush=True)
         assert np.median(u_err) < 1e-6 and np.quantile(u_err, 0.999) < 1e-5 and u_err.max() < 1e-4
+
+
+def test_scale_workload_at_4m(cs, orc):
+    """The scale workload at its own size, N = 4,194,304 (turbulent sphere,
+    rms Mach 2, 64 +- 8 neighbours, eps = 0.001): the mixed form's pairwise
+    gravity for 1024 sampled particles against the FP64 oracle's direct sum
+    over all 4M (max 1e-5, median 1e-7); the multipole far field against the
+    pairwise result (max 1e-5); one SPH force evaluation conserving momentum
+    to rounding, with list lengths in the configuration's 64 +- 8 band."""
+    n = 1 << 22
+    print(f"scale workload n={n}", flush=True)
+    e = cs.make_engine("mixed", 5, n, 12345, ph_sort=True, i_order=0)
+    e.run_pass(cs.GRAVITY)
+    g_pair = e.get("grav")[:n].astype(np.float64)
+    e.set_scalar("grav_far", 1)
+    e.run_pass(cs.GRAVITY)
+    g_far = e.get("grav")[:n].astype(np.float64)
+    e.set_scalar("grav_far", 0)
+    e.run_pass(cs.SPH_ACC)
+    bd = e.get("bdots")[:n].astype(np.float64)
+    cnt = e.get("ngb_count")[:n]
+    pos64, vel64 = e.get("pos64")[:n].copy(), e.get("vel64")[:n].copy()
+    p = e.params.copy()
+    e.close()
+    print(f"  mean list length {cnt.mean():.2f}", flush=True)
+    assert 56 <= cnt.mean() <= 72
+    m = vel64[:, 3]
+    total = (m[:, None] * bd[:, :3]).sum(0)
+    scale = (m[:, None] * np.abs(bd[:, :3])).sum(0)
+    assert np.all(np.abs(total) <= 1e-5 * scale), (total, scale)
+    rms = np.sqrt((g_pair[:, :3] ** 2).sum(1).mean())
+    far_err = np.linalg.norm(g_far[:, :3] - g_pair[:, :3], axis=1) / np.maximum(
+        np.linalg.norm(g_pair[:, :3], axis=1), 1e-3 * rms)
+    print(f"  far field vs pairwise: median {np.median(far_err):.2e} max {far_err.max():.2e}", flush=True)
+    assert far_err.max() < 1e-5
+    p.n_threads = 16
+    windows = np.linspace(0, n - 128, 8).astype(np.int64)
+    idx, ref = sampled_fp64_gravity(orc, p, pos64, vel64, windows, 128)
+    err, perr = gravity_errors(g_pair[idx], ref)
+    print(f"  pairwise vs FP64 oracle: median {np.median(err):.2e} max {err.max():.2e} "
+          f"potential max {perr.max():.2e}", flush=True)
+    assert err.max() < 1e-5 and np.median(err) < 1e-7
+    assert perr.max() < 2e-6
