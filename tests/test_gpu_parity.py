@@ -179,6 +179,22 @@ def test_plummer_ten_steps_lists_bit_exact(cs, orc):
         assert_bits(e.get(name)[:n], o.get(name)[:n], name)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("cfg,n", [(3, 4099), (2, 1037), (1, 513)])
+def test_plain_gravity_bit_exact_ragged(cs, orc, cfg, n, mode):
+    """Gravity alone at sizes that leave a partial LDS tile with an odd
+    record count and a partial block of i-particles, spline and Plummer
+    softening: bit-identical to the oracle."""
+    print(f"plain gravity cfg={cfg} n={n} mode={mode}", flush=True)
+    e, o, _ = make_pair(cs, orc, cfg, n, grav_mode=mode)
+    e.set_scalar("exact_rsqrt", 1)
+    o.init()
+    copy_state(e, o)
+    e.run_pass(cs.GRAVITY)
+    o.run_pass(orc.GRAVITY)
+    assert_bits(e.get("grav")[:n], o.get("grav")[:n], "grav")
+
+
 @pytest.mark.parametrize("cfg,n", [(2, 16384), (3, 16384)])
 def test_plain_gravity_hw_rsqrt(cs, orc, cfg, n):
     """The bench's gravity (hardware rsqrt, as the reference's rsqrtf) against
