@@ -44,6 +44,17 @@ PEAK_FP32_TFLOPS = 157.3
 FLOP_PER_PAIR = 20
 WORKLOAD = "two n=1.5 polytropes (R=1, M=0.5 each), parabolic orbit q=0.5 from separation 5; " \
            "gas+radiation EOS mu=0.6 a=1e-3; 64+-8 neighbours; Plummer eps=0.002"
+# the seeded workloads of `--config` (3, the default, is the headline)
+WORKLOADS = {
+    1: "uniform cube [0,1)^3, m=1/N, v=0, u=1; 32+-4 neighbours; Plummer eps=0.01",
+    2: "Plummer sphere (M=1, a=1, r<10), velocities from its distribution function, u for 10% of the "
+       "virial support; 50+-5 neighbours; Plummer eps=0.005",
+    3: WORKLOAD,
+    4: "Plummer sphere as config 2 at N=65,536; 50+-5 neighbours; Plummer eps=0.005",
+    5: "uniform sphere R=1, M=1, Gaussian velocity field P(k)~k^-4 on 128^3 at rms Mach 2, u=1; "
+       "64+-8 neighbours; Plummer eps=0.001",
+}
+CONFIG = 3
 
 
 def progress(msg):
@@ -54,7 +65,7 @@ def progress(msg):
 def run_form(cs, form, n, seed, warmup, steps, timing=True, i_order=0, scalars=None):
     # the initial conditions are put in Peano-Hilbert order (the reference's
     # periodic sort_bodies) so neighbour gathers read nearby memory
-    e = cs.make_engine(form, 3, n, seed, ph_sort=True, i_order=i_order)
+    e = cs.make_engine(form, CONFIG, n, seed, ph_sort=True, i_order=i_order)
     e.set_scalar("timing", 1 if timing else 0)
     for k, v in (scalars or {}).items():
         e.set_scalar(k, v)
@@ -162,9 +173,10 @@ def cpu_step(cs, n, seed, threads, gravity_sample):
     integration on all N, the N^2 gravity on `gravity_sample` i-particles
     against all N j, scaled to N.  Returns (seconds per step, parts)."""
     import oracle
-    p = cs.preset_params(3, n)
+    p = cs.preset_params(CONFIG, n)
     p.n_threads = threads
-    pos, vel, hyd = cs.generate_ic(3, n, seed)
+    pos, vel, hyd = cs.generate_ic(CONFIG, n, seed)
+    gravity_sample = min(gravity_sample, p.n_dev)
     o = oracle.Oracle(32, p)
     o.load_ic(pos, vel, hyd)
     t0 = time.perf_counter()
@@ -366,7 +378,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--n", type=int, default=0, help="particles (default: the config's own N)")
+    ap.add_argument("--config", type=int, default=3, choices=(1, 2, 3, 4, 5),
+                    help="the seeded workload; 3 (the default) is the headline")
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -404,7 +418,9 @@ def main():
         sys.exit(2)
 
     import cudasph_amd as cs
-    n, K, W = args.n, args.steps, args.warmup
+    global CONFIG
+    CONFIG = args.config
+    n, K, W = args.n or cs.preset_params(CONFIG, 0).n_bodies, args.steps, args.warmup
     if args.scale:
         print(json.dumps(scale_leg(cs, args.scale_n, args.scale_steps, args.seed)))
         return
@@ -435,7 +451,7 @@ def main():
     value = n * K / mixed["wall_s"]
     g_ms, g_calls = mixed["timers"].get("gravity", (float("nan"), 0))
     g_avg = g_ms / max(g_calls, 1) / 1e3
-    nd = cs.preset_params(3, n).n_dev
+    nd = cs.preset_params(CONFIG, n).n_dev
     achieved = FLOP_PER_PAIR * float(nd) * float(nd) / g_avg / 1e12
 
     def drift(run):
@@ -454,7 +470,8 @@ def main():
                 "angular": abs(run["angular_momentum"] - fp64["angular_momentum"]) / abs(fp64["angular_momentum"])}
 
     line = {
-        "metric": "particle-updates/s @1M (mixed vs plain FP32, rel err <=1e-6); adjoint/fwd ratio",
+        "metric": ("particle-updates/s @1M (mixed vs plain FP32, rel err <=1e-6); adjoint/fwd ratio" if CONFIG == 3
+                   else f"particle-updates/s, config {CONFIG} (mixed vs plain FP32)"),
         "value": value,
         "unit": "particle-updates/s",
         "n_gpus": 1,
@@ -466,7 +483,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded initial conditions generated in-process)",
-        "config": {"workload": WORKLOAD, "n": n, "form": "mixed", "seed": args.seed,
+        "config": {"workload": WORKLOADS[CONFIG], "n": n, "form": "mixed", "seed": args.seed,
                    "initial_order": "Peano-Hilbert", "i_order": ["memory", "kd-tree heap"][args.i_order]},
         "plain_fp32": None if plain is None else {
             "value": n * K / plain["wall_s"], "ms_per_step": plain["wall_s"] / K * 1e3,
@@ -487,7 +504,7 @@ def main():
                                             mixed["kbar"], mixed["walks"])},
         "roofline": {"bound": "valu", "kernel": "k_gravity_mixed_plummer",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profile("k_gravity"),
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profile("k_gravity") if CONFIG == 3 else None,
                      "frac_of_measured_fma": achieved / peaks["fp32_fma_TFLOPs"],
                      "frac_of_pair_body": achieved / peaks["gravity_pair_body_TFLOPs"],
                      "per_launch": f"{FLOP_PER_PAIR} flop x {nd}^2 pairs",
