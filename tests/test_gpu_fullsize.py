@@ -14,18 +14,33 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("cfg,n", [(2, 262144), (3, 1048576)])
 def test_sph_force_evaluation_bit_exact_at_full_size(cs, orc, cfg, n):
-    print(f"full-size SPH evaluation cfg={cfg} n={n}", flush=True)
+    """One force evaluation from the initial state: ranges, counts, offsets,
+    walk lengths, every list slot in walk order and the SPH outputs
+    bit-identical to the oracle's."""
     p = cs.preset_params(cfg, n)
     pos, vel, hyd = cs.generate_ic(cfg, n, 12345)
-    e = cs.Engine("plain", p)
-    e.load_ic(pos, vel, hyd)
-    e.run_pass(cs.SPH_ACC)
     p.n_threads = 16
     o = orc.Oracle(32, p)
     o.load_ic(pos, vel, hyd)
     o.run_pass(orc.SPH_ACC)
-    print("  oracle done", flush=True)
+    print(f"full-size SPH evaluation cfg={cfg} n={n}: oracle done", flush=True)
+    e = cs.Engine("plain", p)
+    e.load_ic(pos, vel, hyd)
+    e.run_pass(cs.SPH_ACC)
+    check_force_evaluation(e, o, p, n, vel[:, 3])
+
+
+def walk_lengths(sim, n):
+    """nodes each particle's list walk visited (left in the hash keys)"""
+    h = sim.get("hash")
+    out = np.zeros(len(h), np.int64)
+    out[h[:, 1]] = h[:, 0]
+    return out[:n]
+
+
+def check_force_evaluation(e, o, p, n, m):
     assert_bits(e.get("pos")[:n], o.get("pos")[:n], "pos (ranges)")
+    assert np.array_equal(walk_lengths(e, n), walk_lengths(o, n)), "walk lengths differ"
     assert np.array_equal(e.get("n_ngb")[:n], o.get("n_ngb")[:n])
     cnt = e.get("ngb_count")
     assert np.array_equal(cnt[:n], o.get("ngb_count")[:n])
@@ -43,7 +58,7 @@ def test_sph_force_evaluation_bit_exact_at_full_size(cs, orc, cfg, n):
     assert_bits(e.get("hydro2")[:n, 0], o.get("hydro2")[:n, 0], "v_sig")
     # hydro + reaction conserve momentum to rounding
     bd = e.get("bdots")[:n].astype(np.float64)
-    m = vel[:n, 3]
+    m = m[:n]
     total = (m[:, None] * bd[:, :3]).sum(0)
     scale = (m[:, None] * np.abs(bd[:, :3])).sum(0)
     assert np.all(np.abs(total) <= 1e-5 * scale), (total, scale)

@@ -37,6 +37,8 @@ def walk_lengths(sim):
 
 @pytest.mark.parametrize("cfg,n", CASES)
 def test_plain_passes_bit_exact(cs, orc, cfg, n):
+    """Each pass from the oracle's state: bit-identical outputs, lists slot by
+    slot, walk lengths."""
     print(f"plain passes cfg={cfg} n={n}", flush=True)
     e, o, p = make_pair(cs, orc, cfg, n)
     e.set_scalar("exact_rsqrt", 1)
