@@ -86,10 +86,14 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
   CS_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
   cap_ = 100L * nd_;  // ngb_tot_max (read_state.cpp:29)
   tree_gpu_ = form == SPH_FORM_MIXED;   // the plain and FP64 forms keep the reference-exact host build
+  // the plain form walks exactly as the reference (its walk lengths are an
+  // output, bodies_hash.key); the others prune by subtree boxes
+  walk_boxes_ = form != SPH_FORM_PLAIN;
 
   for (auto* b : {&pos_, &vel_, &hydro_, &dots_, &bdots_, &hydro2_, &grav_, &gpos_, &gvel_, &old_vu_, &tmp4_})
     b->alloc(nd_);
   tree_.alloc(nd_ + 2);
+  boxes_.alloc(2 * ((size_t)nd_ + 2));
   hash_.alloc(nd_);
   perm_.alloc(nd_);
   n_ngb_.alloc(nd_);
@@ -176,7 +180,10 @@ long Engine<T>::set(const std::string& name, const void* host, long nb) {
   if (name == "bdots") return put(bdots_, bdots_.bytes());
   if (name == "hydro2") return put(hydro2_, hydro2_.bytes());
   if (name == "grav") return put(grav_, grav_.bytes());
-  if (name == "tree") return put(tree_, tree_.bytes());
+  if (name == "tree") {
+    boxes_stale_ = true;
+    return put(tree_, tree_.bytes());
+  }
   if (name == "hash") return put(hash_, hash_.bytes());
   if (name == "n_ngb") return put(n_ngb_, n_ngb_.bytes());
   if (name == "ngb_count") return put(ngb_count_, ngb_count_.bytes());
@@ -256,6 +263,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_far_ratio") return grav_far_ratio_;
   if (name == "grav_far_grade") return grav_far_grade_;
   if (name == "grav_far_half") return grav_far_half_;
+  if (name == "walk_boxes") return walk_boxes_;
   if (name == "overlap") return overlap_;
   if (name == "tree_gpu") return tree_gpu_;
   if (name == "timing") return timers_.enabled;
@@ -299,6 +307,8 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
   } else if (name == "grav_far_grade") {
     if (!(v >= 0 && v <= 0.2)) throw std::invalid_argument("grav_far_grade must lie in [0, 0.2]");
     grav_far_grade_ = v;
+  } else if (name == "walk_boxes") {
+    walk_boxes_ = v != 0;
   } else if (name == "overlap") {
     overlap_ = (int)v;
   } else if (name == "tree_gpu") {
@@ -381,6 +391,7 @@ void Engine<T>::tree_fetch() {
     TimerScope ts(timers_, "tree_build", s_);
     launch_tree_build<T>(pos_, n_, nd_, tree_, hash_, i_order_, tree_work_, pairs_, pairs_alt_, sort_tmp_, scan_tmp_,
                          s_);
+    boxes_stale_ = true;
   } else {
     CS_HIP(hipMemcpyAsync(h_pos_.p, pos_, (size_t)nd_ * sizeof(V4<T>), hipMemcpyDeviceToHost, s_));
   }
@@ -400,6 +411,7 @@ void Engine<T>::tree_build() {
   timers_.add_host("tree_build_host",
                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   CS_HIP(hipMemcpyAsync(tree_, h_tree_.p, (size_t)(nd_ + 2) * sizeof(V4<T>), hipMemcpyHostToDevice, s_));
+  boxes_stale_ = true;
   CS_HIP(hipMemcpyAsync(hash_, h_hash_.p, (size_t)nd_ * sizeof(KV), hipMemcpyHostToDevice, s_));
 }
 
@@ -407,6 +419,15 @@ template <class T>
 void Engine<T>::tree() {
   tree_fetch();
   tree_build();
+}
+
+// the subtree boxes of the current tree, when the walks use them
+template <class T>
+void Engine<T>::ensure_boxes() {
+  if (!walk_boxes_ || !boxes_stale_) return;
+  TimerScope ts(timers_, "tree_boxes", s_);
+  launch_tree_boxes<T>(tree_, boxes_, p_.n_bodies, p_.fix_phantom, s_);
+  boxes_stale_ = false;
 }
 
 template <class T>
@@ -420,6 +441,7 @@ void Engine<T>::solve() {
     CS_HIP(hipMemcpyAsync(n_ngb_, adj_nngb_.p + rec, (size_t)nd_ * 4, hipMemcpyDeviceToDevice, s_));
     return;
   }
+  ensure_boxes();
   {
     TimerScope ts(timers_, "solve_range", s_);
     launch_solve_range<T>(pos_, n_ngb_, hash_, tree_, nd_, walk_params(), s_);
@@ -459,6 +481,7 @@ long Engine<T>::check_total() {
 // counts and walk again
 template <class T>
 void Engine<T>::find() {
+  ensure_boxes();
   const WalkParams wp = walk_params();
   if (tot_stale_) check_total();   // offsets set from outside since the last scan
   for (int attempt = 0; attempt < 2; ++attempt) {

@@ -67,8 +67,10 @@ class Engine final : public EngineBase {
  private:
   static constexpr bool kF32 = sizeof(T) == 4;
   bool mixed() const { return form_ == SPH_FORM_MIXED; }
+  void ensure_boxes();
   WalkParams walk_params() const {
-    return {p_.n_bodies, p_.fix_phantom, p_.rel_err, p_.desired_ngb, p_.max_iter};
+    return {p_.n_bodies, p_.fix_phantom, p_.rel_err, p_.desired_ngb, p_.max_iter,
+            walk_boxes_ ? boxes_.p : nullptr};
   }
   EosConst eos_const() const {
     EosConst c;
@@ -189,6 +191,8 @@ class Engine final : public EngineBase {
   DBuf<D4> pos64_, vel64_, old64_;
   DBuf<double> u64_;
   DBuf<F4> pos_lo_, ghi_, glo_, gvel32_, gpart_;
+  DBuf<F4> boxes_;              // subtree bounding boxes of tree_ (launch_tree_boxes), 2 per node
+  bool boxes_stale_ = true;     // tree_ changed since they were built
   DBuf<float> gmass_;   // {min, max} of the masses + reduction partials (mixed gravity)
   DBuf<float> gmom_;    // per-sub-tile multipole records (mixed gravity far field)
   bool dirty64_ = false;
@@ -214,6 +218,7 @@ class Engine final : public EngineBase {
   // ties; with ties the same neighbour sets; the mixed form's default);
   // 0: on the host, bit-identical to the reference's quicksort build
   int tree_gpu_ = 0;
+  int walk_boxes_ = 0;   // walks prune subtrees by their bounding boxes (mixed and FP64 forms)
   DBuf<int> tree_work_;   // allocated on first use of the device build
   long ngb_tot_ = 0;
   bool tot_stale_ = true;   // ngb_tot_ not yet read for the current offsets (a host round trip)

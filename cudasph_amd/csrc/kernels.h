@@ -12,6 +12,10 @@ struct WalkParams {
   float rel_err;
   float desired_ngb;
   int max_iter;
+  // subtree bounding boxes (launch_tree_boxes), or null: with them the walk
+  // also skips a child whose subtree box lies beyond the range.  The same
+  // neighbours in the same order; fewer nodes visited (the walk length)
+  const F4* boxes = nullptr;
 };
 
 struct EosConst;  // sph_math.h
@@ -105,6 +109,11 @@ long tree_work_ints(long n);
 template <class T>
 void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash, int heap_order, int* work,
                        KV* kv, KV* kv_alt, int* sort_scratch, int* scan_scratch, hipStream_t s);
+// Subtree bounding boxes of a kd-tree over n_nodes particles: box[2k] (low
+// corner) and box[2k+1] (high corner) of node k's subtree, for k up to
+// n_nodes+1; FP32, rounded outward.  One launch per tree level.
+template <class T>
+void launch_tree_boxes(const V4<T>* tree, F4* box, int n_nodes, int fix_phantom, hipStream_t s);
 
 // ---- integrate.hip ----
 template <class T>

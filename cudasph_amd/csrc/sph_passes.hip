@@ -28,10 +28,26 @@ namespace cudasph {
 // given, bit 0 of *phantom is set when the walk reads node n+1 and bit 1 when
 // that read passes the distance test (a spurious visit of particle 0 when the
 // zero record sits there, n = n_dev).
+//
+// With `boxes` (subtree bounding boxes, launch_tree_boxes) a child is pushed
+// only if its box also comes within the range: a subtree whose box lies
+// beyond it holds no neighbour, so the neighbours, their order and every sum
+// over them stay the same, and only the walk length shrinks.  The children's
+// boxes (64 adjacent bytes) are fetched together with the node, so the chain
+// of dependent fetches does not grow.  The test keeps a margin of 2^-16 of
+// r^2 over the rounding of the box distance and of d^2.
+template <class T>
+__device__ __forceinline__ bool box_within(const F4 lo, const F4 hi, T qx, T qy, T qz, T r2m) {
+  const T dx = fmax(fmax(T(lo.x) - qx, qx - T(hi.x)), T(0));
+  const T dy = fmax(fmax(T(lo.y) - qy, qy - T(hi.y)), T(0));
+  const T dz = fmax(fmax(T(lo.z) - qz, qz - T(hi.z)), T(0));
+  return dx * dx + dy * dy + dz * dz <= r2m;
+}
+
 template <class T, class Visit>
 __device__ __forceinline__ int kd_walk(const V4<T> q, const V4<T>* __restrict__ tree, int n_nodes,
                                        int fix_phantom, int* __restrict__ stk, Visit&& visit,
-                                       int* phantom = nullptr) {
+                                       int* phantom = nullptr, const F4* __restrict__ boxes = nullptr) {
   int top = 0;
   stk[0] = 1;
   int n_walk = 0;
@@ -39,19 +55,29 @@ __device__ __forceinline__ int kd_walk(const V4<T> q, const V4<T>* __restrict__ 
   // becomes a select of addresses, which keeps the caller's copy in scratch
   const T qx = q.x, qy = q.y, qz = q.z, qw = q.w;
   const T r2 = qw * qw;
+  const T r2m = r2 * T(1.0 + 1.0 / 65536.0);
   while (top >= 0) {
     ++n_walk;
     int id = stk[top * kBlock];
     const bool ph = phantom && id == n_nodes + 1;
     const V4<T> node = tree[id];
+    F4 b0, b1, b2, b3;
+    if (boxes) {   // children's boxes, beside the node fetch (a leaf fetches the root's: unused)
+      const int cb = 2 * id <= n_nodes ? 4 * id : 4;
+      b0 = boxes[cb];
+      b1 = boxes[cb + 1];
+      b2 = boxes[cb + 2];
+      b3 = boxes[cb + 3];
+    }
     const int bits = node_bits(node.w);
     const int split = (bits >> 24) & 3;
     id <<= 1;
     if (id <= n_nodes && split < 3) {
       const T qc = (split & 2) ? qz : ((split & 1) ? qy : qx);
       const T nc = (split & 2) ? node.z : ((split & 1) ? node.y : node.x);
-      if (qc - qw < nc) stk[(top++) * kBlock] = id;
-      if (qc + qw > nc && (!fix_phantom || id < n_nodes)) stk[(top++) * kBlock] = id + 1;
+      if (qc - qw < nc && (!boxes || box_within<T>(b0, b1, qx, qy, qz, r2m))) stk[(top++) * kBlock] = id;
+      if (qc + qw > nc && (!fix_phantom || id < n_nodes) && (!boxes || box_within<T>(b2, b3, qx, qy, qz, r2m)))
+        stk[(top++) * kBlock] = id + 1;
     }
     --top;
     const T dx = qx - node.x, dy = qy - node.y, dz = qz - node.z;
@@ -78,7 +104,7 @@ __device__ __forceinline__ NgbEval<T> eval_ngb(const V4<T>& q, const V4<T>* __re
     f += kern_ig<T>(u);
     df -= kern_digdh<T>(u);
     cnt += T(1);
-  });
+  }, nullptr, wp.boxes);
   f -= T(wp.desired_ngb);
   df = T((double)df * (1.0 / (double)q.w));
   if (df != T(0)) df = f / df;
@@ -206,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_find_ngb(const V4<T>* __restrict__ p
       list[off + c] = id;
     }
     ++c;
-  }, &ph);
+  }, &ph, wp.boxes);
   if (vec) {   // the last, partial group (slots past the count stay untouched)
     const int g = c & ~3, r = c & 3;
     if (r > 0 && g < cap) {

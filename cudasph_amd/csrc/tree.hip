@@ -255,7 +255,73 @@ void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash
   }
 }
 
+namespace {
+
+// a coordinate as a float bound: rounded down (lo) or up (hi) when it is not
+// a float already
+__device__ __forceinline__ float lo_bound(float x) { return x; }
+__device__ __forceinline__ float hi_bound(float x) { return x; }
+// the neighbouring float below / above a finite f (its bit pattern moved by
+// one unit away from or towards zero)
+__device__ __forceinline__ float float_below(float f) {
+  const int b = __float_as_int(f);
+  return f > 0.f ? __int_as_float(b - 1) : (f < 0.f ? __int_as_float(b + 1) : -__int_as_float(1));
+}
+__device__ __forceinline__ float float_above(float f) {
+  const int b = __float_as_int(f);
+  return f > 0.f ? __int_as_float(b + 1) : (f < 0.f ? __int_as_float(b - 1) : __int_as_float(1));
+}
+__device__ __forceinline__ float lo_bound(double x) {
+  const float f = (float)x;
+  return (double)f > x ? float_below(f) : f;
+}
+__device__ __forceinline__ float hi_bound(double x) {
+  const float f = (float)x;
+  return (double)f < x ? float_above(f) : f;
+}
+
+// boxes of the nodes first..last (one tree level): the node's own particle
+// and the boxes of the children a walk can enter from it
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_tree_boxes(const V4<T>* __restrict__ tree, F4* __restrict__ box,
+                                                        int n_nodes, int fix_phantom, int first, int last) {
+  const int k = first + blockIdx.x * kBlock + threadIdx.x;
+  if (k > last) return;
+  const V4<T> nd = tree[k];
+  F4 lo = {lo_bound(nd.x), lo_bound(nd.y), lo_bound(nd.z), 0.f};
+  F4 hi = {hi_bound(nd.x), hi_bound(nd.y), hi_bound(nd.z), 0.f};
+  const int c = 2 * k;
+  for (int ch = c; ch <= c + 1; ++ch) {
+    // children 2k, 2k+1 <= n_nodes; node n_nodes+1 (the phantom) is entered
+    // from 2k = n_nodes unless fix_phantom
+    if (ch > n_nodes && !(ch == n_nodes + 1 && c == n_nodes && !fix_phantom)) continue;
+    const F4 a = box[2 * ch], b = box[2 * ch + 1];
+    lo = {fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), 0.f};
+    hi = {fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), 0.f};
+  }
+  box[2 * k] = lo;
+  box[2 * k + 1] = hi;
+}
+
+}  // namespace
+
+template <class T>
+void launch_tree_boxes(const V4<T>* tree, F4* box, int n_nodes, int fix_phantom, hipStream_t s) {
+  const int last_node = n_nodes + 1;   // the phantom's box is its record's point
+  int levels = 0;
+  while ((2l << levels) <= (long)last_node) ++levels;   // deepest level: floor(log2(last_node))
+  for (int L = levels; L >= 0; --L) {
+    const int first = 1 << L;
+    const int last = (int)std::min<long>((2l << L) - 1, (long)last_node);
+    hipLaunchKernelGGL(k_tree_boxes<T>, dim3(ceil_div(last - first + 1, kBlock)), dim3(kBlock), 0, s, tree, box,
+                       n_nodes, fix_phantom, first, last);
+    CS_CHECK_LAUNCH();
+  }
+}
+
 template void launch_tree_build<float>(const F4*, int, int, F4*, KV*, int, int*, KV*, KV*, int*, int*, hipStream_t);
 template void launch_tree_build<double>(const D4*, int, int, D4*, KV*, int, int*, KV*, KV*, int*, int*, hipStream_t);
+template void launch_tree_boxes<float>(const F4*, F4*, int, int, hipStream_t);
+template void launch_tree_boxes<double>(const D4*, F4*, int, int, hipStream_t);
 
 }  // namespace cudasph

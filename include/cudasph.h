@@ -199,7 +199,12 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
  * tree_gpu (1: the kd-tree built on the device, the mixed form's default --
  * the same tree as the host build when no two particles share a coordinate,
  * otherwise the same neighbour sets; 0: the host build, bit-identical to
- * kd_tree.cpp:39-128, the plain and FP64 forms' default).
+ * kd_tree.cpp:39-128, the plain and FP64 forms' default);
+ * walk_boxes (1, the mixed and FP64 forms' default: the tree walks also skip
+ * a child whose subtree bounding box lies beyond the range -- the same
+ * neighbours in the same order, so the same results, with fewer nodes
+ * visited; 0, the plain form's default: the reference's walk, whose node
+ * counts it leaves in bodies_hash.key).
  * Mixed-form gravity measurement switches, all results within the form's
  * tolerance: grav_split_only (1: split hi/lo differences for every pair),
  * grav_unit_mass (0: the general-mass body even when every mass is equal),
@@ -236,7 +241,7 @@ enum {
   SPH_STAT_NGB_MEAN, SPH_STAT_NGB_SIGMA, SPH_STAT_NGB_MIN, SPH_STAT_NGB_MAX,  /* h-solve counts */
   SPH_STAT_COUNT_MISMATCH,   /* particles whose list length differs from the h-solve count */
   SPH_STAT_MAX_COUNT,        /* longest neighbour list                                      */
-  SPH_STAT_PHANTOM_READS,    /* walks of the last find_ngb that read node n+1               */
+  SPH_STAT_PHANTOM_READS,    /* walks of the last find_ngb that read node n+1 (fewer with walk_boxes) */
   SPH_STAT_PHANTOM_NGB,      /* ... and took its record as a neighbour (only when n = n_dev) */
   SPH_STAT_OVERFLOW_RERUNS,  /* find_ngb re-walks after a slot overflow, since creation     */
   SPH_STAT_ITERATION, SPH_STAT_TIME,

@@ -369,6 +369,42 @@ def test_gravity_stream_overlap_changes_nothing(cs, form, far):
         assert_bits(out[1][k], out[0][k], k)
 
 
+def walk_total(e):
+    """nodes visited by the last list pass, summed over particles (the walk
+    lengths it leaves in the hash keys)"""
+    return int(e.get("hash")[:, 0].astype(np.int64).sum())
+
+
+@pytest.mark.parametrize("form,cfg,n", [("mixed", 3, 16384), ("mixed", 2, 16000), ("mixed", 1, 16001),
+                                        ("fp64", 2, 16384), ("fp64", 3, 16001)])
+def test_box_pruned_walks_change_nothing(cs, form, cfg, n):
+    """The mixed and FP64 forms' tree walks skip subtrees whose bounding box
+    lies beyond the range.  Three steps with and without the boxes give the
+    same bits everywhere (ranges, counts, every list slot in order, the
+    state), while the pruned walks visit fewer nodes.  n = 16384 = n_dev puts
+    the zero record at node n+1; 16000 (even) makes node n+1 reachable; 16001
+    (odd) does not."""
+    print(f"box-pruned walks {form} cfg={cfg} n={n}", flush=True)
+    out, walks = [], []
+    for boxes in (0, 1):
+        e = cs.make_engine(form, cfg, n, 23, ph_sort=True)
+        assert e.scalar("walk_boxes") == 1
+        e.set_scalar("walk_boxes", boxes)
+        e.init()
+        e.step(3)
+        r = {k: e.get(k)[:n].copy() for k in ("pos", "vel", "hydro", "dots", "bdots", "grav", "n_ngb", "ngb_count")}
+        off = e.get("ngb_offset")
+        r["ngb_offset"] = off.copy()
+        r["ngb_list"] = e.get("ngb_list")[: int(off[-1])].copy()
+        out.append(r)
+        walks.append(walk_total(e))
+        e.close()
+    for k in out[0]:
+        assert_bits(out[1][k], out[0][k], k)
+    print(f"  nodes visited by the list pass: {walks[0]} -> {walks[1]}", flush=True)
+    assert walks[1] < walks[0]
+
+
 def test_state_file_round_trip(cs, tmp_path):
     n = 5000
     e = cs.make_engine("plain", 2, n, 3)

@@ -65,6 +65,31 @@ def check_force_evaluation(e, o, p, n, m):
     print(f"  mean list length {cnt[:n].mean():.2f}, ngb_tot {tot}", flush=True)
 
 
+@pytest.mark.parametrize("cfg", [2, 3])
+def test_box_pruned_walks_change_nothing_at_full_size(cs, cfg):
+    """The mixed form with and without subtree-box pruning of its walks, one
+    step from the same state at the config's full size: the same bits in the
+    ranges, counts, every list slot and the state."""
+    n = cs.preset_params(cfg, 0).n_bodies
+    print(f"box-pruned walks at full size cfg={cfg} n={n}", flush=True)
+    out, walks = [], []
+    for boxes in (0, 1):
+        e = cs.make_engine("mixed", cfg, n, 12345, ph_sort=True)
+        e.set_scalar("walk_boxes", boxes)
+        e.init()
+        e.step()
+        r = {k: e.get(k)[:n].copy() for k in ("pos", "vel", "hydro", "bdots", "n_ngb", "ngb_count")}
+        off = e.get("ngb_offset")
+        r["ngb_list"] = e.get("ngb_list")[: int(off[-1])].copy()
+        out.append(r)
+        walks.append(float(e.get("hash")[:, 0].astype(np.int64).sum()) / n)
+        e.close()
+    for k in out[0]:
+        assert_bits(out[1][k], out[0][k], k)
+    print(f"  nodes per list walk: {walks[0]:.1f} -> {walks[1]:.1f}", flush=True)
+    assert walks[1] < walks[0]
+
+
 def test_device_tree_same_neighbour_sets_at_full_size(cs):
     """At N = 2^20 the 24-bit coordinate draws tie many times per axis, so the
     device tree build (the mixed form's default) may pick other tied
