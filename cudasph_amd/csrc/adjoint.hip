@@ -395,14 +395,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(const D4v* __restrict__ 
 
 // Plummer gravity reversed: a_i = -sum_j m_j r_ij / s^3, s^2 = |r_ij|^2 + eps^2
 //   x_bar_i += sum_j T_ij (m_i a_bar_j - m_j a_bar_i),  T = I/s^3 - 3 r r^T / s^5
-// One FP64 N^2 pass: j-tiles of (x, m, a_bar) staged in LDS, one i per lane.
+// One FP64 N^2 pass: j-tiles of (x, m, a_bar) staged in LDS, one i per lane,
+// the pair body in fused multiply-adds.  As in the forward FP64 kernel, with
+// few i-blocks the j-range is split into gravity64_parts(n) parts whose sums
+// (in `part`) are added in part order.
 constexpr int kAdjTile = kBlock;
 __global__ __launch_bounds__(kBlock) void k_adj_gravity(const D4v* __restrict__ pos, const D4v* __restrict__ vel,
-                                                         const D4v* __restrict__ abar, D4v* __restrict__ xs, int n,
+                                                         const D4v* __restrict__ abar, D4v* __restrict__ xs,
+                                                         D4v* __restrict__ part, int n, int parts, int chunk,
                                                          double eps2) {
   __shared__ double sx[kAdjTile], sy[kAdjTile], sz[kAdjTile], sm[kAdjTile];
   __shared__ double sax[kAdjTile], say[kAdjTile], saz[kAdjTile];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int pt = (int)blockIdx.x % parts, ib = (int)blockIdx.x / parts;
+  const int i = ib * kBlock + threadIdx.x;
   D4v pi{}, ai{};
   double mi = 0.0;
   if (i < n) {
@@ -410,11 +415,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_gravity(const D4v* __restrict__ 
     ai = abar[i];
     mi = vel[i].w;
   }
+  const int j_begin = pt * chunk, j_end = min(n, j_begin + chunk);
   double gx = 0, gy = 0, gz = 0;
-  for (int j0 = 0; j0 < n; j0 += kAdjTile) {
+  for (int j0 = j_begin; j0 < j_end; j0 += kAdjTile) {
     __syncthreads();
     const int j = j0 + threadIdx.x;
-    if (j < n) {
+    if (j < j_end) {
       const D4v p = pos[j], a = abar[j];
       sx[threadIdx.x] = p.x;
       sy[threadIdx.x] = p.y;
@@ -423,36 +429,54 @@ __global__ __launch_bounds__(kBlock) void k_adj_gravity(const D4v* __restrict__ 
       sax[threadIdx.x] = a.x;
       say[threadIdx.x] = a.y;
       saz[threadIdx.x] = a.z;
-    } else {
-      sx[threadIdx.x] = sy[threadIdx.x] = sz[threadIdx.x] = 0.0;
-      sm[threadIdx.x] = sax[threadIdx.x] = say[threadIdx.x] = saz[threadIdx.x] = 0.0;
     }
     __syncthreads();
-    const int m = min(kAdjTile, n - j0);
-    auto body = [&](int jj) {
+    const int m = min(kAdjTile, j_end - j0);
+    for (int jj = 0; jj < m; ++jj) {
       const double rx = pi.x - sx[jj], ry = pi.y - sy[jj], rz = pi.z - sz[jj];
-      const double d2 = rx * rx + ry * ry + rz * rz + eps2;
+      const double d2 = __builtin_fma(rx, rx, __builtin_fma(ry, ry, __builtin_fma(rz, rz, eps2)));
       double inv = __builtin_amdgcn_rsq(d2);
-      inv = inv * (1.5 - 0.5 * d2 * inv * inv);   // one Newton step: ~1e-14 relative
+      inv = inv * __builtin_fma(-0.5 * d2, inv * inv, 1.5);   // one Newton step: ~1e-14 relative
       const double inv2 = inv * inv, inv3 = inv2 * inv, inv5 = inv3 * inv2;
-      const double bx = mi * sax[jj] - sm[jj] * ai.x;
-      const double by = mi * say[jj] - sm[jj] * ai.y;
-      const double bz = mi * saz[jj] - sm[jj] * ai.z;
-      const double t = 3.0 * (rx * bx + ry * by + rz * bz) * inv5;
-      gx += bx * inv3 - t * rx;
-      gy += by * inv3 - t * ry;
-      gz += bz * inv3 - t * rz;
-    };
-    // (unrolling this loop measured slower: the FP64 body is long enough)
-    for (int jj = 0; jj < m; ++jj) body(jj);
+      const double mj = sm[jj];
+      const double bx = __builtin_fma(mi, sax[jj], -mj * ai.x);
+      const double by = __builtin_fma(mi, say[jj], -mj * ai.y);
+      const double bz = __builtin_fma(mi, saz[jj], -mj * ai.z);
+      const double t = 3.0 * __builtin_fma(rx, bx, __builtin_fma(ry, by, rz * bz)) * inv5;
+      gx = __builtin_fma(bx, inv3, __builtin_fma(-t, rx, gx));
+      gy = __builtin_fma(by, inv3, __builtin_fma(-t, ry, gy));
+      gz = __builtin_fma(bz, inv3, __builtin_fma(-t, rz, gz));
+    }
   }
-  if (i < n) {
-    D4v x = xs[i];
-    x.x += gx;
-    x.y += gy;
-    x.z += gz;
-    xs[i] = x;
+  if (i >= n) return;
+  if (parts > 1) {
+    part[(size_t)pt * n + i] = {gx, gy, gz, 0.0};
+    return;
   }
+  D4v x = xs[i];
+  x.x += gx;
+  x.y += gy;
+  x.z += gz;
+  xs[i] = x;
+}
+
+// the parts' sums in part order, added to x_bar
+__global__ __launch_bounds__(kBlock) void k_adj_gravity_combine(const D4v* __restrict__ part, D4v* __restrict__ xs,
+                                                                 int n, int parts) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double gx = 0, gy = 0, gz = 0;
+  for (int p = 0; p < parts; ++p) {
+    const D4v g = part[(size_t)p * n + i];
+    gx += g.x;
+    gy += g.y;
+    gz += g.z;
+  }
+  D4v x = xs[i];
+  x.x += gx;
+  x.y += gy;
+  x.z += gz;
+  xs[i] = x;
 }
 
 // corrector reversed: a_bar += v_bar f, udot_bar += u_bar f
@@ -909,8 +933,16 @@ void Engine<T>::adj_forces_backward(bool seed_rho) {
     ts.t.end(s_);
     ts.t.begin("adjoint_gravity", s_);
     const double eps = (double)p_.grav_eps;
-    hipLaunchKernelGGL(k_adj_gravity, grid(n_), dim3(kBlock), 0, s_, pos_.p, vel_.p, aa_.p, xs_.p, n_, eps * eps);
+    const int parts = gravity64_parts(n_);
+    const int chunk = ceil_div(ceil_div(n_, parts), kAdjTile) * kAdjTile;
+    hipLaunchKernelGGL(k_adj_gravity, dim3(ceil_div(n_, kBlock) * parts), dim3(kBlock), 0, s_, pos_.p, vel_.p, aa_.p,
+                       xs_.p, (D4v*)gpart64_.p, n_, parts, chunk, eps * eps);
     CS_CHECK_LAUNCH();
+    if (parts > 1) {
+      hipLaunchKernelGGL(k_adj_gravity_combine, grid(n_), dim3(kBlock), 0, s_, (const D4v*)gpart64_.p, xs_.p, n_,
+                         parts);
+      CS_CHECK_LAUNCH();
+    }
   }
 }
 

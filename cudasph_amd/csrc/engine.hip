@@ -85,7 +85,9 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
   CS_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
   CS_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
   cap_ = 100L * nd_;  // ngb_tot_max (read_state.cpp:29)
-  tree_gpu_ = form == SPH_FORM_MIXED;   // the plain and FP64 forms keep the reference-exact host build
+  // the plain form keeps the reference-exact host build (its lists follow
+  // the reference's tie-breaking); the others build on the device
+  tree_gpu_ = form != SPH_FORM_PLAIN;
   // the plain form walks exactly as the reference (its walk lengths are an
   // output, bodies_hash.key); the others prune by subtree boxes
   walk_boxes_ = form != SPH_FORM_PLAIN;
@@ -94,6 +96,7 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
     b->alloc(nd_);
   tree_.alloc(nd_ + 2);
   boxes_.alloc(2 * ((size_t)nd_ + 2));
+  if (!kF32 && gravity64_parts(nd_) > 1) gpart64_.alloc((size_t)gravity64_parts(nd_) * nd_);
   hash_.alloc(nd_);
   perm_.alloc(nd_);
   n_ngb_.alloc(nd_);
@@ -651,6 +654,12 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
       return;
     }
   }
+  if constexpr (!kF32) {
+    if (p_.grav_mode == 1) {
+      launch_gravity64_plummer(gpos_, grav_, gpart64_.p, nd_, nd_, (double)p_.grav_eps, gs);
+      return;
+    }
+  }
   launch_gravity<T>(gpos_, gvel_, grav_, nd_, nd_, p_.grav_mode, T(p_.grav_eps), exact_rsqrt_, gs);
 }
 
@@ -791,10 +800,11 @@ void Engine<T>::step() {
   tree_fetch();            // also reads back dt_
   // overlap < 0 (the default): on for the mixed form, whose SPH passes take
   // ~12 ms at N = 2^20 and slow its gravity by ~4 ms when they run beside it
-  // (288.5 -> 282.9 ms/step; with the far field 106.2 -> 101.6), off for the
-  // plain and FP64 forms, whose longer h-solve walks lose more than they hide
-  // (plain: 1057 -> 1215 ms/step)
-  const bool overlap = overlap_ >= 0 ? overlap_ != 0 : mixed();
+  // (288.5 -> 282.9 ms/step; with the far field 106.2 -> 101.6), and for the
+  // FP64 form (827 -> 814 ms/step at 2^20, 58.2 -> 53.4 at 262 144, 6.80 ->
+  // 6.70 at 65 536); off for the plain form, whose host tree build and
+  // unpruned h-solve walks lose more than they hide (1057 -> 1215 ms/step)
+  const bool overlap = overlap_ >= 0 ? overlap_ != 0 : form_ != SPH_FORM_PLAIN;
   if (grav_first) gravity(overlap);
   tree_build();
   sph_after_tree();

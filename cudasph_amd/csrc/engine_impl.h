@@ -189,6 +189,7 @@ class Engine final : public EngineBase {
   DBuf<uint32_t> key_hi_, key_lo_;
   // mixed form
   DBuf<D4> pos64_, vel64_, old64_;
+  DBuf<D4> gpart64_;            // FP64 form: per-j-part gravity sums (launch_gravity64_plummer)
   DBuf<double> u64_;
   DBuf<F4> pos_lo_, ghi_, glo_, gvel32_, gpart_;
   DBuf<F4> boxes_;              // subtree bounding boxes of tree_ (launch_tree_boxes), 2 per node

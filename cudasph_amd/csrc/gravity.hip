@@ -926,6 +926,79 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_mixed_spline(const F4* _
   if (i < n_i) out[i] = {a[0].get(), a[1].get(), a[2].get(), a[3].get()};
 }
 
+// ---------------------------------------------------------------------------
+// FP64 form, Plummer softening.  The pair body in fused multiply-adds (the
+// FP64 form has no bit contract with the reference's FP32 arithmetic):
+// 3 sub, 3 FMA for s^2 = r^2 + eps^2, v_rsq_f64 and two Newton steps
+// (y <- y (1.5 - (s^2/2) y^2), 3 operations each after s^2/2), 3 for
+// m/s^3, 3 FMA into the acceleration and one into the potential.  With few
+// i-blocks (N below 2^19) the j-range is split into parts, one block per
+// (i-block, part), so that the GPU holds enough wavefronts to hide the FP64
+// latencies; the parts' sums are added in part order (deterministic).
+constexpr int kG64Tile = kGravBlock;
+__global__ __launch_bounds__(kGravBlock) void k_gravity64_plummer(const D4* __restrict__ gpos, D4* __restrict__ out,
+                                                                   int n_i, int n_j, int parts, int chunk,
+                                                                   double eps2) {
+  __shared__ D4 s_pos[kG64Tile];
+  const int part = (int)blockIdx.x % parts, ib = (int)blockIdx.x / parts;
+  const int i = ib * kGravBlock + threadIdx.x;
+  const D4 pi = gpos[i < n_i ? i : n_i - 1];
+  const int j_begin = part * chunk, j_end = min(n_j, j_begin + chunk);
+  double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
+  for (int t0 = j_begin; t0 < j_end; t0 += kG64Tile) {
+    const int cnt = min(kG64Tile, j_end - t0);
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) s_pos[threadIdx.x] = gpos[t0 + threadIdx.x];
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const D4 pj = s_pos[j];
+      const double rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+      const double d2 = __builtin_fma(rx, rx, __builtin_fma(ry, ry, __builtin_fma(rz, rz, eps2)));
+      const double h = 0.5 * d2;
+      double y = __builtin_amdgcn_rsq(d2);
+      y = y * __builtin_fma(-h, y * y, 1.5);
+      y = y * __builtin_fma(-h, y * y, 1.5);
+      const double my = pj.w * y;
+      const double s3 = my * (y * y);
+      ax = __builtin_fma(-rx, s3, ax);
+      ay = __builtin_fma(-ry, s3, ay);
+      az = __builtin_fma(-rz, s3, az);
+      ph -= my;
+    }
+  }
+  if (i < n_i) out[(size_t)part * n_i + i] = {ax, ay, az, ph};
+}
+
+// the parts' sums, in part order
+__global__ __launch_bounds__(kBlock) void k_gravity64_combine(const D4* __restrict__ part, D4* __restrict__ out,
+                                                               int n_i, int parts) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n_i) return;
+  D4 a = part[i];
+  for (int p = 1; p < parts; ++p) {
+    const D4 b = part[(size_t)p * n_i + i];
+    a = {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w};
+  }
+  out[i] = a;
+}
+
+int gravity64_parts(int n_i) {
+  const int blocks = ceil_div(n_i, kGravBlock);
+  return std::max(1, std::min(8, ceil_div(2048, blocks)));
+}
+
+void launch_gravity64_plummer(const D4* gpos, D4* out, D4* part, int n_i, int n_j, double eps, hipStream_t s) {
+  const int parts = gravity64_parts(n_i);
+  const int chunk = ceil_div(ceil_div(n_j, parts), kG64Tile) * kG64Tile;
+  hipLaunchKernelGGL(k_gravity64_plummer, dim3(ceil_div(n_i, kGravBlock) * parts), dim3(kGravBlock), 0, s, gpos,
+                     parts > 1 ? part : out, n_i, n_j, parts, chunk, eps * eps);
+  CS_CHECK_LAUNCH();
+  if (parts > 1) {
+    hipLaunchKernelGGL(k_gravity64_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part, out, n_i, parts);
+    CS_CHECK_LAUNCH();
+  }
+}
+
 template <class T>
 void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, int n_j, int soft, T eps,
                     int exact, hipStream_t s) {

@@ -349,7 +349,7 @@ def test_mixed_far_field_trajectory_against_fp64(cs, orc):
     assert np.abs(x - xo).max() <= 1e-6 * np.abs(xo).max()
 
 
-@pytest.mark.parametrize("form,far", [("plain", 0), ("mixed", 0), ("mixed", 1)])
+@pytest.mark.parametrize("form,far", [("plain", 0), ("mixed", 0), ("mixed", 1), ("fp64", 0)])
 def test_gravity_stream_overlap_changes_nothing(cs, form, far):
     """Gravity on its own stream beside the SPH passes gives the same bits as
     the serial order (with the far field, the default overlap -1 takes the
