@@ -797,7 +797,6 @@ void Engine<T>::step() {
   // (sph_accelerations, then CUDA_gravity, step.cpp:65,94).  The spline
   // softening needs the new ranges and keeps that order.
   const bool grav_first = p_.grav_mode == 1;
-  tree_fetch();            // also reads back dt_
   // overlap < 0 (the default): on for the mixed form, whose SPH passes take
   // ~12 ms at N = 2^20 and slow its gravity by ~4 ms when they run beside it
   // (288.5 -> 282.9 ms/step; with the far field 106.2 -> 101.6), and for the
@@ -805,7 +804,12 @@ void Engine<T>::step() {
   // 6.70 at 65 536); off for the plain form, whose host tree build and
   // unpruned h-solve walks lose more than they hide (1057 -> 1215 ms/step)
   const bool overlap = overlap_ >= 0 ? overlap_ != 0 : form_ != SPH_FORM_PLAIN;
-  if (grav_first) gravity(overlap);
+  // on its own stream, gravity starts before the device tree build, which
+  // then runs beside it (the build is on the SPH passes' path, not gravity's)
+  const bool grav_now = grav_first && overlap && tree_gpu_;
+  if (grav_now) gravity(true);
+  tree_fetch();            // also reads back dt_
+  if (grav_first && !grav_now) gravity(overlap);
   tree_build();
   sph_after_tree();
   if (adj_capture_ >= 0) adj_capture(*adj_eval_[adj_capture_]);
