@@ -40,7 +40,7 @@ namespace cudasph {
 namespace {
 
 // nodes in the subtree of heap node t of the complete tree of n nodes
-__device__ __forceinline__ int subtree_size(int t, int n) {
+__host__ __device__ __forceinline__ int subtree_size(int t, int n) {
   int size = 0;
   long long lo = t, hi = t;
   while (lo <= n) {
@@ -53,7 +53,7 @@ __device__ __forceinline__ int subtree_size(int t, int n) {
 
 // among the nodes of levels >= L, laid out subtree by subtree, the offset of
 // the subtree of level-L node t: the nodes below the level-L nodes left of t
-__device__ __forceinline__ int subtree_start(int t, int L, int n) {
+__host__ __device__ __forceinline__ int subtree_start(int t, int L, int n) {
   int s = 0;
   long long a = 1ll << L, b = t;
   while (a <= n) {
@@ -194,6 +194,165 @@ __global__ __launch_bounds__(kBlock) void k_tree_regroup(const int* __restrict__
   regroup_one(lc, lc_next, sc, at_c, tag, idx, L, n);
 }
 
+// ---------------------------------------------------------------------------
+// The last levels in LDS.  Once every subtree of level L0 holds at most
+// kLdsMax particles, one block per subtree builds the rest of it in its own
+// LDS: the block reads its three runs (particle ids, in the level-L0 layout),
+// names each particle by its rank in the first run (a slot), and runs the
+// split / flags / scans / regroup of the level kernels above on slots, with
+// per-level positions taken relative to where the subtree's runs start in
+// that level's layout.  The level kernels' dependent global-memory passes
+// (the build's cost at small N) become LDS passes; the results are the same
+// nodes in the same places.
+constexpr int kLdsMax = 2048;
+
+struct LocalLevel {
+  int L, r0, r1, r0n;        // level, the runs' range in its layout, and their start at level L+1
+  int t_base;                // first level-L node of the subtree
+};
+
+__device__ __forceinline__ int block_excl_scan(int v, int* s_tot) {
+  // exclusive scan of one value per thread (kBlock threads), through LDS
+  const int tid = threadIdx.x;
+  s_tot[tid] = v;
+  __syncthreads();
+  for (int o = 1; o < kBlock; o <<= 1) {
+    const int add = tid >= o ? s_tot[tid - o] : 0;
+    __syncthreads();
+    s_tot[tid] += add;
+    __syncthreads();
+  }
+  const int incl = s_tot[tid];
+  __syncthreads();
+  return incl - v;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_tree_lds(const int* __restrict__ work, int* __restrict__ slot_of, int n,
+                                                      int L0, int cur0, int cur1, int cur2,
+                                                      const V4<T>* __restrict__ pos, V4<T>* __restrict__ tree,
+                                                      KV* __restrict__ hash, int heap_order) {
+  __shared__ uint16_t s_list[3][2][kLdsMax];   // slots, per axis, double-buffered
+  __shared__ int s_tag[kLdsMax];               // the heap node each slot's particle belongs to
+  __shared__ int s_id[kLdsMax];                // slot -> particle id
+  __shared__ uint8_t s_fb[kLdsMax], s_fc[kLdsMax];
+  __shared__ uint16_t s_sb[kLdsMax], s_sc[kLdsMax];
+  __shared__ uint16_t s_at_b[kLdsMax], s_at_c[kLdsMax];   // local position of each node's placed particle
+  __shared__ int s_tot[kBlock];
+  const int tid = threadIdx.x;
+  const int T0 = (1 << L0) + (int)blockIdx.x;
+  if (T0 > n) return;
+  const int base0 = subtree_start(T0, L0, n);
+  const int m0 = subtree_size(T0, n);
+  const int* gl[3];
+  const int curs[3] = {cur0, cur1, cur2};
+  for (int a = 0; a < 3; ++a) gl[a] = work + (size_t)(2 * a + curs[a]) * n + base0;
+  // slots: rank in the first run; slot_of (indexed by particle) carries
+  // the names to the other two runs
+  for (int k = tid; k < m0; k += kBlock) {
+    const int id = gl[0][k];
+    s_id[k] = id;
+    s_tag[k] = T0;
+    s_list[0][0][k] = (uint16_t)k;
+    slot_of[id] = k;
+  }
+  __syncthreads();
+  for (int k = tid; k < m0; k += kBlock) {
+    s_list[1][0][k] = (uint16_t)slot_of[gl[1][k]];
+    s_list[2][0][k] = (uint16_t)slot_of[gl[2][k]];
+  }
+  __syncthreads();
+  int cur[3] = {0, 0, 0};
+  for (int L = L0; L < 31; ++L) {
+    const long long d0 = (long long)T0 << (L - L0), d1 = (long long)(T0 + 1) << (L - L0);
+    if (d0 > n) break;
+    const int r0 = subtree_start((int)d0, L, n);
+    const int r1 = subtree_start((int)min(d1, (long long)n + 1), L, n);
+    if (r0 >= r1) break;
+    const long long e0 = d0 << 1;
+    const int r0n = subtree_start((int)min(e0, (long long)n + 1), L + 1, n);
+    const int a = L % 3, b = (L + 1) % 3, c = (L + 2) % 3;
+    const int next_level = 2 << L;
+    const int t_base = (int)d0;
+    // split on axis a
+    for (int i = tid; i < r1 - r0; i += kBlock) {
+      const int g = r0 + i;
+      const int slot = s_list[a][cur[a]][i];
+      const int t = s_tag[slot];
+      const int p = g - subtree_start(t, L, n);
+      const int lt = 2 * t <= n ? subtree_size(2 * t, n) : 0;
+      const int pb = t - (1 << L);
+      if (p < lt) {
+        s_tag[slot] = 2 * t;
+        s_list[a][cur[a] ^ 1][g - pb - r0n] = (uint16_t)slot;
+      } else if (p == lt) {
+        const int id = s_id[slot];
+        const V4<T> q = pos[id];
+        tree[t] = {q.x, q.y, q.z, node_word(T(0), id | (a << 24))};
+        if (heap_order) hash[t - 1] = {(uint32_t)(t - 1), (uint32_t)id};
+      } else {
+        s_tag[slot] = 2 * t + 1;
+        s_list[a][cur[a] ^ 1][g - pb - 1 - r0n] = (uint16_t)slot;
+      }
+    }
+    __syncthreads();
+    // flags of the other two lists, and where each node's placed particle sits in them
+    for (int i = tid; i < r1 - r0; i += kBlock) {
+      const int tb = s_tag[s_list[b][cur[b]][i]], tc = s_tag[s_list[c][cur[c]][i]];
+      s_fb[i] = tb >= next_level && !(tb & 1);
+      s_fc[i] = tc >= next_level && !(tc & 1);
+      if (tb < next_level) s_at_b[tb - t_base] = (uint16_t)i;
+      if (tc < next_level) s_at_c[tc - t_base] = (uint16_t)i;
+    }
+    __syncthreads();
+    // exclusive scans of the flags over the runs (chunks of ceil(R / kBlock) per thread)
+    {
+      const int R = r1 - r0;
+      const int chunk = (R + kBlock - 1) / kBlock;
+      const int i0 = min(R, tid * chunk), i1 = min(R, i0 + chunk);
+      int cb = 0, cc = 0;
+      for (int i = i0; i < i1; ++i) {
+        cb += s_fb[i];
+        cc += s_fc[i];
+      }
+      int ob = block_excl_scan(cb, s_tot);
+      int oc = block_excl_scan(cc, s_tot);
+      for (int i = i0; i < i1; ++i) {
+        s_sb[i] = (uint16_t)ob;
+        s_sc[i] = (uint16_t)oc;
+        ob += s_fb[i];
+        oc += s_fc[i];
+      }
+    }
+    __syncthreads();
+    // regroup the other two lists into the next level's runs
+    for (int i = tid; i < r1 - r0; i += kBlock) {
+      const int g = r0 + i;
+      for (int k = 0; k < 2; ++k) {
+        const int ax = k ? c : b;
+        const int slot = s_list[ax][cur[ax]][i];
+        const int nt = s_tag[slot];
+        if (nt < next_level) continue;   // placed at this level
+        const int t = nt >> 1;
+        const int s = subtree_start(t, L, n);
+        const int base = s - (t - (1 << L));
+        const uint16_t* scn = k ? s_sc : s_sb;
+        const int lefts = scn[i] - scn[s - r0];
+        int dst;
+        if (!(nt & 1)) {
+          dst = base + lefts;
+        } else {
+          const int at = (k ? s_at_c : s_at_b)[t - t_base];
+          dst = base + subtree_size(2 * t, n) + (g - s) - lefts - (at < i ? 1 : 0);
+        }
+        s_list[ax][cur[ax] ^ 1][dst - r0n] = (uint16_t)slot;
+      }
+    }
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) cur[k] ^= 1;
+  }
+}
+
 }  // namespace
 
 long tree_work_ints(long n) { return 13 * n + 8; }
@@ -233,7 +392,11 @@ void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash
   }
   int cur[3] = {0, 0, 0};
   int m = n;
-  for (int L = 0; m > 0; ++L) {
+  // level kernels while the level's subtrees hold more than kLdsMax
+  // particles (the leftmost is the largest), then one launch in LDS
+  int L_lds = 0;
+  while (subtree_size(1 << L_lds, n) > kLdsMax) ++L_lds;
+  for (int L = 0; m > 0 && L < L_lds; ++L) {
     const int a = L % 3, b = (L + 1) % 3, c = (L + 2) % 3;
     const dim3 gm(ceil_div(m, kBlock));
     hipLaunchKernelGGL((k_tree_split<T>), gm, dim3(kBlock), 0, s, list[a][cur[a]], list[a][cur[a] ^ 1], tag, pos,
@@ -252,6 +415,12 @@ void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash
     }
     for (int k = 0; k < 3; ++k) cur[k] ^= 1;
     m -= placed;
+  }
+  if (m > 0) {
+    const int roots = (int)std::min<long>(1l << L_lds, (long)n - (1l << L_lds) + 1);
+    hipLaunchKernelGGL((k_tree_lds<T>), dim3(roots), dim3(kBlock), 0, s, work, tag, n, L_lds, cur[0], cur[1], cur[2],
+                       pos, tree, hash, heap_order);
+    CS_CHECK_LAUNCH();
   }
 }
 
