@@ -81,6 +81,11 @@ void launch_exclusive_scan(const int* in, int* out, long n, int mode, int* scrat
 // radix_scratch_ints(n) ints.
 long radix_scratch_ints(long n);
 void launch_radix_sort(KV* data, KV* alt, long n, int bits, int* scratch, hipStream_t s);
+// `batch` independent sorts of n records in one sequence of launches: sort b
+// on data + b * data_stride (alt likewise) with scratch + b *
+// radix_scratch_ints(n)
+void launch_radix_sort_batch(KV* data, KV* alt, long n, int bits, int* scratch, hipStream_t s, int batch,
+                             long data_stride);
 // the reaction lists: for each of the n particles j, the list slots k
 // naming j in ascending k, as pairs {key = owner i, value = k} in
 // pairs[start[j], end[j]) (end[j] = start[j + 1]; start holds n + 1 ints).
@@ -108,8 +113,8 @@ void launch_ph_keys(const V4<T>* pos, int n, int bits, uint32_t* key_hi, uint32_
 // The kd-tree of the first n positions (kd_tree.cpp:39-128) built on the
 // device: tree[0..n_dev+1] and hash[0..n_dev) as the host builder writes them
 // (heap_order 1: hash in the nodes' heap order, 0: identity).  work holds
-// tree_work_ints(n) ints; kv and kv_alt n records each; sort_scratch
-// radix_scratch_ints(n) and scan_scratch scan_scratch_ints(n) ints.
+// tree_work_ints(n) ints; kv and kv_alt 3 n records each; sort_scratch
+// 3 radix_scratch_ints(n) and scan_scratch scan_scratch_ints(n) ints.
 long tree_work_ints(long n);
 template <class T>
 void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash, int heap_order, int* work,

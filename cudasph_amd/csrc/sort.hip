@@ -43,9 +43,14 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_tmp, int& tota
   return incl - v;
 }
 
+// Batched scans (blockIdx.y = batch b): array b's input, output and tile
+// partials start `stride` ints after array b-1's (one stride: each batch
+// lives at the same offsets inside its own stretch of memory).
 __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int* __restrict__ in, long n, int mode,
-                                                             int* __restrict__ partial) {
+                                                             int* __restrict__ partial, long stride) {
   __shared__ int s_tmp[kScanBlock];
+  in += blockIdx.y * stride;
+  partial += blockIdx.y * stride;
   const long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
   int sum = 0;
   for (int k = 0; k < kScanItems; ++k) sum += scan_load(in, base + k, n, mode);
@@ -55,8 +60,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int* __restric
 }
 
 // one block scans the tile partials in chunks of 256, carrying the sum
-__global__ __launch_bounds__(kScanBlock) void k_scan_partials(int* __restrict__ partial, int n_part) {
+__global__ __launch_bounds__(kScanBlock) void k_scan_partials(int* __restrict__ partial, int n_part, long stride) {
   __shared__ int s_tmp[kScanBlock];
+  partial += blockIdx.y * stride;
   int carry = 0;
   for (int c0 = 0; c0 < n_part; c0 += kScanBlock) {
     const int idx = c0 + threadIdx.x;
@@ -71,8 +77,11 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(int* __restrict__ 
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int* __restrict__ in, long n, int mode,
                                                            const int* __restrict__ partial, int* __restrict__ out,
-                                                           int n_part) {
+                                                           int n_part, long stride) {
   __shared__ int s_tmp[kScanBlock];
+  in += blockIdx.y * stride;
+  partial += blockIdx.y * stride;
+  out += blockIdx.y * stride;
   const long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
   int v[kScanItems];
   int sum = 0;
@@ -91,18 +100,26 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int* __restrict_
 
 long scan_scratch_ints(long n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
-void launch_exclusive_scan(const int* in, int* out, long n, int mode, int* scratch, hipStream_t s) {
+// `batch` scans at once: in, out and scratch of scan b at b * stride ints
+// past the first's
+static void exclusive_scan_batch(const int* in, int* out, long n, int mode, int* scratch, hipStream_t s,
+                                 int batch, long stride) {
   const int n_part = (int)((n + kScanTile - 1) / kScanTile);
   if (n_part == 0) {
-    CS_HIP(hipMemsetAsync(out, 0, sizeof(int), s));
+    for (int b = 0; b < batch; ++b) CS_HIP(hipMemsetAsync(out + b * stride, 0, sizeof(int), s));
     return;
   }
-  hipLaunchKernelGGL(k_scan_reduce, dim3(n_part), dim3(kScanBlock), 0, s, in, n, mode, scratch);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(n_part, batch), dim3(kScanBlock), 0, s, in, n, mode, scratch, stride);
   CS_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlock), 0, s, scratch, n_part);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1, batch), dim3(kScanBlock), 0, s, scratch, n_part, stride);
   CS_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_scan_down, dim3(n_part), dim3(kScanBlock), 0, s, in, n, mode, scratch, out, n_part);
+  hipLaunchKernelGGL(k_scan_down, dim3(n_part, batch), dim3(kScanBlock), 0, s, in, n, mode, scratch, out, n_part,
+                     stride);
   CS_CHECK_LAUNCH();
+}
+
+void launch_exclusive_scan(const int* in, int* out, long n, int mode, int* scratch, hipStream_t s) {
+  exclusive_scan_batch(in, out, n, mode, scratch, s, 1, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -112,8 +129,11 @@ constexpr int kRadixRounds = 16;
 constexpr int kRadixTile = kRadixBlock * kRadixRounds;   // 4096 records per block
 
 __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const KV* __restrict__ in, long n, int shift,
-                                                             int* __restrict__ counts, int n_blocks) {
+                                                             int* __restrict__ counts, int n_blocks,
+                                                             long data_stride, long scratch_stride) {
   __shared__ int s_hist[256];
+  in += blockIdx.y * data_stride;
+  counts += blockIdx.y * scratch_stride;
   s_hist[threadIdx.x] = 0;
   __syncthreads();
   const long base = (long)blockIdx.x * kRadixTile;
@@ -127,8 +147,12 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const KV* __restrict
 
 __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const KV* __restrict__ in, KV* __restrict__ out,
                                                                 long n, int shift,
-                                                                const int* __restrict__ offsets, int n_blocks) {
+                                                                const int* __restrict__ offsets, int n_blocks,
+                                                                long data_stride, long scratch_stride) {
   __shared__ int s_base[256];
+  in += blockIdx.y * data_stride;
+  out += blockIdx.y * data_stride;
+  offsets += blockIdx.y * scratch_stride;
   __shared__ int s_wave[4][256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   s_base[tid] = offsets[(long)tid * n_blocks + blockIdx.x];
@@ -174,10 +198,12 @@ long radix_scratch_ints(long n) {
   return 2 * table + 2 + scan_scratch_ints(table);
 }
 
-void launch_radix_sort(KV* data, KV* alt, long n, int bits, int* scratch, hipStream_t s) {
-  if (n <= 1 || bits <= 0) return;
+void launch_radix_sort_batch(KV* data, KV* alt, long n, int bits, int* scratch, hipStream_t s, int batch,
+                             long data_stride) {
+  if (n <= 1 || bits <= 0 || batch <= 0) return;
   const int nb = (int)((n + kRadixTile - 1) / kRadixTile);
   const long table = 256L * nb;
+  const long sstride = radix_scratch_ints(n);   // each sort's scratch
   int* counts = scratch;
   int* offsets = scratch + table;
   int* scan_tmp = scratch + 2 * table + 2;
@@ -185,16 +211,27 @@ void launch_radix_sort(KV* data, KV* alt, long n, int bits, int* scratch, hipStr
   KV* dst = alt;
   int passes = 0;
   for (int shift = 0; shift < bits; shift += 8, ++passes) {
-    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kRadixBlock), 0, s, src, n, shift, counts, nb);
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb, batch), dim3(kRadixBlock), 0, s, src, n, shift, counts, nb,
+                       data_stride, sstride);
     CS_CHECK_LAUNCH();
-    launch_exclusive_scan(counts, offsets, table, 0, scan_tmp, s);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kRadixBlock), 0, s, src, dst, n, shift, offsets, nb);
+    // the scan's input, output and partials sit at the same offsets in
+    // every sort's scratch
+    exclusive_scan_batch(counts, offsets, table, 0, scan_tmp, s, batch, sstride);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nb, batch), dim3(kRadixBlock), 0, s, src, dst, n, shift, offsets, nb,
+                       data_stride, sstride);
     CS_CHECK_LAUNCH();
     KV* t = src;
     src = dst;
     dst = t;
   }
-  if (passes & 1) CS_HIP(hipMemcpyAsync(data, alt, n * sizeof(KV), hipMemcpyDeviceToDevice, s));
+  if (passes & 1)
+    for (int b = 0; b < batch; ++b)
+      CS_HIP(hipMemcpyAsync(data + b * data_stride, alt + b * data_stride, n * sizeof(KV), hipMemcpyDeviceToDevice,
+                            s));
+}
+
+void launch_radix_sort(KV* data, KV* alt, long n, int bits, int* scratch, hipStream_t s) {
+  launch_radix_sort_batch(data, alt, n, bits, scratch, s, 1, 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -88,20 +88,24 @@ __device__ __forceinline__ double node_word(double, int bits) { return (double)b
 // pass re-keys records already ordered by the first)
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_tree_keys(const V4<T>* __restrict__ pos, KV* __restrict__ kv, int n,
-                                                       int axis, int word, int first) {
+                                                       int word, int first) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int axis = blockIdx.y;   // the three axes' records one after another in kv
+  kv += (size_t)axis * n;
   if (i >= n) return;
   const int id = first ? i : (int)kv[i].value;
   const auto b = ordered_bits(coord(pos[id], axis));
   kv[i] = {(uint32_t)(word ? (uint64_t)b >> 32 : (uint64_t)b & 0xffffffffu), (uint32_t)id};
 }
 
+// axis blockIdx.y's sorted ids into its list (the lists 2 n ints apart)
 __global__ __launch_bounds__(kBlock) void k_tree_ids(const KV* __restrict__ kv, int* __restrict__ list,
-                                                      int* __restrict__ tag, int n, int set_tag) {
+                                                      int* __restrict__ tag, int n) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int axis = blockIdx.y;
   if (i >= n) return;
-  list[i] = (int)kv[i].value;
-  if (set_tag) tag[i] = 1;
+  list[(size_t)axis * 2 * n + i] = (int)kv[(size_t)axis * n + i].value;
+  if (axis == 0) tag[i] = 1;
 }
 
 // fixed slots (sph_accelerations.cpp:21-33, read_state.cpp:85-88): slot 0 and
@@ -375,21 +379,19 @@ void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash
   int* sc = work + 10 * (size_t)n + 1;
   int* at_b = work + 11 * (size_t)n + 2 - 1;   // indexed by heap node 1..n
   int* at_c = work + 12 * (size_t)n + 2 - 1;
-  const dim3 g(ceil_div(n, kBlock));
-  // the three presorted lists (FP64 coordinates: low key word, then high)
-  for (int a = 0; a < 3; ++a) {
-    const bool wide = sizeof(T) == 8;
-    hipLaunchKernelGGL((k_tree_keys<T>), g, dim3(kBlock), 0, s, pos, kv, n, a, 0, 1);
+  // the three presorted lists, sorted side by side (FP64 coordinates: low
+  // key word, then high)
+  const dim3 g3(ceil_div(n, kBlock), 3);
+  hipLaunchKernelGGL((k_tree_keys<T>), g3, dim3(kBlock), 0, s, pos, kv, n, 0, 1);
+  CS_CHECK_LAUNCH();
+  launch_radix_sort_batch(kv, kv_alt, n, 32, sort_scratch, s, 3, n);
+  if (sizeof(T) == 8) {
+    hipLaunchKernelGGL((k_tree_keys<T>), g3, dim3(kBlock), 0, s, pos, kv, n, 1, 0);
     CS_CHECK_LAUNCH();
-    launch_radix_sort(kv, kv_alt, n, 32, sort_scratch, s);
-    if (wide) {
-      hipLaunchKernelGGL((k_tree_keys<T>), g, dim3(kBlock), 0, s, pos, kv, n, a, 1, 0);
-      CS_CHECK_LAUNCH();
-      launch_radix_sort(kv, kv_alt, n, 32, sort_scratch, s);
-    }
-    hipLaunchKernelGGL(k_tree_ids, g, dim3(kBlock), 0, s, kv, list[a][0], tag, n, a == 0);
-    CS_CHECK_LAUNCH();
+    launch_radix_sort_batch(kv, kv_alt, n, 32, sort_scratch, s, 3, n);
   }
+  hipLaunchKernelGGL(k_tree_ids, g3, dim3(kBlock), 0, s, kv, list[0][0], tag, n);
+  CS_CHECK_LAUNCH();
   int cur[3] = {0, 0, 0};
   int m = n;
   // level kernels while the level's subtrees hold more than kLdsMax
