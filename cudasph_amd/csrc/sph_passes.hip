@@ -51,6 +51,7 @@ __device__ __forceinline__ int kd_walk(const V4<T> q, const V4<T>* __restrict__ 
   int top = 0;
   stk[0] = 1;
   int n_walk = 0;
+  const int stride = (int)blockDim.x;   // the stack: one LDS column per lane of the block
   // the query by value, in scalars: selecting a component of a reference
   // becomes a select of addresses, which keeps the caller's copy in scratch
   const T qx = q.x, qy = q.y, qz = q.z, qw = q.w;
@@ -58,7 +59,7 @@ __device__ __forceinline__ int kd_walk(const V4<T> q, const V4<T>* __restrict__ 
   const T r2m = r2 * T(1.0 + 1.0 / 65536.0);
   while (top >= 0) {
     ++n_walk;
-    int id = stk[top * kBlock];
+    int id = stk[top * stride];
     const bool ph = phantom && id == n_nodes + 1;
     const V4<T> node = tree[id];
     F4 b0, b1, b2, b3;
@@ -75,9 +76,9 @@ __device__ __forceinline__ int kd_walk(const V4<T> q, const V4<T>* __restrict__ 
     if (id <= n_nodes && split < 3) {
       const T qc = (split & 2) ? qz : ((split & 1) ? qy : qx);
       const T nc = (split & 2) ? node.z : ((split & 1) ? node.y : node.x);
-      if (qc - qw < nc && (!boxes || box_within<T>(b0, b1, qx, qy, qz, r2m))) stk[(top++) * kBlock] = id;
+      if (qc - qw < nc && (!boxes || box_within<T>(b0, b1, qx, qy, qz, r2m))) stk[(top++) * stride] = id;
       if (qc + qw > nc && (!fix_phantom || id < n_nodes) && (!boxes || box_within<T>(b2, b3, qx, qy, qz, r2m)))
-        stk[(top++) * kBlock] = id + 1;
+        stk[(top++) * stride] = id + 1;
     }
     --top;
     const T dx = qx - node.x, dy = qy - node.y, dz = qz - node.z;
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_range(V4<T>* __restrict__ pos,
                                                          const V4<T>* __restrict__ tree, int n_dev,
                                                          WalkParams wp) {
   extern __shared__ int s_stack[];   // walk_stack_depth(n_nodes) entries per lane
-  const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * (int)blockDim.x + threadIdx.x;
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
   const V4<T> q = pos[i];
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_find_ngb(const V4<T>* __restrict__ p
                                                       int* __restrict__ list, int* __restrict__ flags,
                                                       int n_dev, WalkParams wp, int clamp) {
   extern __shared__ int s_stack[];   // walk_stack_depth(n_nodes) entries per lane
-  const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * (int)blockDim.x + threadIdx.x;
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
   const V4<T> q = pos[i];
@@ -577,12 +578,19 @@ __global__ __launch_bounds__(kBlock) void k_hydro_mixed(const F4* __restrict__ p
 
 // ---------------------------------------------------------------------------
 // launchers
+
+// lanes per block of the tree walks: 256, or 64 when 256-lane blocks would
+// leave CUs empty (below 256 blocks), so that the walks' wavefronts spread
+// over every CU instead of sharing the memory pipelines of a quarter of them
+static int walk_block(int n_dev) { return n_dev < 256 * kBlock ? 64 : kBlock; }
+
 template <class T>
 void launch_solve_range(V4<T>* pos, int* n_ngb, const KV* hash, const V4<T>* tree, int n_dev,
                         const WalkParams& wp, hipStream_t s) {
-  const size_t lds = (size_t)walk_stack_depth(wp.n_nodes) * kBlock * sizeof(int);
-  hipLaunchKernelGGL(k_solve_range<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), lds, s, pos, n_ngb, hash,
-                     tree, n_dev, wp);
+  const int bs = walk_block(n_dev);
+  const size_t lds = (size_t)walk_stack_depth(wp.n_nodes) * bs * sizeof(int);
+  hipLaunchKernelGGL(k_solve_range<T>, dim3(ceil_div(n_dev, bs)), dim3(bs), lds, s, pos, n_ngb, hash, tree, n_dev,
+                     wp);
   CS_CHECK_LAUNCH();
 }
 
@@ -590,9 +598,10 @@ template <class T>
 void launch_find_ngb(const V4<T>* pos, const V4<T>* tree, KV* hash, const int* offset, const int* cap_count,
                      int* count_out, int* list, int* flags, int n_dev, const WalkParams& wp, int clamp,
                      hipStream_t s) {
-  const size_t lds = (size_t)walk_stack_depth(wp.n_nodes) * kBlock * sizeof(int);
-  hipLaunchKernelGGL(k_find_ngb<T>, dim3(ceil_div(n_dev, kBlock)), dim3(kBlock), lds, s, pos, tree, hash, offset,
-                     cap_count, count_out, list, flags, n_dev, wp, clamp);
+  const int bs = walk_block(n_dev);
+  const size_t lds = (size_t)walk_stack_depth(wp.n_nodes) * bs * sizeof(int);
+  hipLaunchKernelGGL(k_find_ngb<T>, dim3(ceil_div(n_dev, bs)), dim3(bs), lds, s, pos, tree, hash, offset, cap_count,
+                     count_out, list, flags, n_dev, wp, clamp);
   CS_CHECK_LAUNCH();
 }
 
