@@ -209,40 +209,53 @@ __global__ __launch_bounds__(kBlock) void k_tree_regroup(const int* __restrict__
 // (the build's cost at small N) become LDS passes; the results are the same
 // nodes in the same places.
 constexpr int kLdsMax = 2048;
+// threads of a finishing block: few blocks run (one per subtree), so each
+// takes a whole CU's worth of wavefronts to keep its passes short
+constexpr int kLdsThreads = 1024;
 
 struct LocalLevel {
   int L, r0, r1, r0n;        // level, the runs' range in its layout, and their start at level L+1
   int t_base;                // first level-L node of the subtree
 };
 
-__device__ __forceinline__ int block_excl_scan(int v, int* s_tot) {
-  // exclusive scan of one value per thread (kBlock threads), through LDS
-  const int tid = threadIdx.x;
-  s_tot[tid] = v;
-  __syncthreads();
-  for (int o = 1; o < kBlock; o <<= 1) {
-    const int add = tid >= o ? s_tot[tid - o] : 0;
-    __syncthreads();
-    s_tot[tid] += add;
-    __syncthreads();
+// exclusive scans of two values per thread (kLdsThreads threads) at once: within
+// each wavefront by shuffles, then across the block's wavefronts through
+// LDS (one barrier)
+__device__ __forceinline__ void block_excl_scan2(int& x, int& y, int* s_tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int ix = x, iy = y;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int ux = __shfl_up(ix, o), uy = __shfl_up(iy, o);
+    if (lane >= o) {
+      ix += ux;
+      iy += uy;
+    }
   }
-  const int incl = s_tot[tid];
+  if (lane == 63) {
+    s_tot[wave] = ix;
+    s_tot[kLdsThreads / 64 + wave] = iy;
+  }
   __syncthreads();
-  return incl - v;
+  int bx = 0, by = 0;
+  for (int w = 0; w < wave; ++w) {
+    bx += s_tot[w];
+    by += s_tot[kLdsThreads / 64 + w];
+  }
+  x = bx + ix - x;
+  y = by + iy - y;
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_tree_lds(const int* __restrict__ work, int* __restrict__ slot_of, int n,
+__global__ __launch_bounds__(kLdsThreads) void k_tree_lds(const int* __restrict__ work, int* __restrict__ slot_of, int n,
                                                       int L0, int cur0, int cur1, int cur2,
                                                       const V4<T>* __restrict__ pos, V4<T>* __restrict__ tree,
                                                       KV* __restrict__ hash, int heap_order) {
   __shared__ uint16_t s_list[3][2][kLdsMax];   // slots, per axis, double-buffered
   __shared__ int s_tag[kLdsMax];               // the heap node each slot's particle belongs to
-  __shared__ int s_id[kLdsMax];                // slot -> particle id
   __shared__ uint8_t s_fb[kLdsMax], s_fc[kLdsMax];
   __shared__ uint16_t s_sb[kLdsMax], s_sc[kLdsMax];
   __shared__ uint16_t s_at_b[kLdsMax], s_at_c[kLdsMax];   // local position of each node's placed particle
-  __shared__ int s_tot[kBlock];
+  __shared__ int s_tot[2 * kLdsThreads / 64];
   const int tid = threadIdx.x;
   const int T0 = (1 << L0) + (int)blockIdx.x;
   if (T0 > n) return;
@@ -253,15 +266,14 @@ __global__ __launch_bounds__(kBlock) void k_tree_lds(const int* __restrict__ wor
   for (int a = 0; a < 3; ++a) gl[a] = work + (size_t)(2 * a + curs[a]) * n + base0;
   // slots: rank in the first run; slot_of (indexed by particle) carries
   // the names to the other two runs
-  for (int k = tid; k < m0; k += kBlock) {
+  for (int k = tid; k < m0; k += kLdsThreads) {
     const int id = gl[0][k];
-    s_id[k] = id;
     s_tag[k] = T0;
     s_list[0][0][k] = (uint16_t)k;
     slot_of[id] = k;
   }
   __syncthreads();
-  for (int k = tid; k < m0; k += kBlock) {
+  for (int k = tid; k < m0; k += kLdsThreads) {
     s_list[1][0][k] = (uint16_t)slot_of[gl[1][k]];
     s_list[2][0][k] = (uint16_t)slot_of[gl[2][k]];
   }
@@ -279,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_tree_lds(const int* __restrict__ wor
     const int next_level = 2 << L;
     const int t_base = (int)d0;
     // split on axis a
-    for (int i = tid; i < r1 - r0; i += kBlock) {
+    for (int i = tid; i < r1 - r0; i += kLdsThreads) {
       const int g = r0 + i;
       const int slot = s_list[a][cur[a]][i];
       const int t = s_tag[slot];
@@ -290,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_tree_lds(const int* __restrict__ wor
         s_tag[slot] = 2 * t;
         s_list[a][cur[a] ^ 1][g - pb - r0n] = (uint16_t)slot;
       } else if (p == lt) {
-        const int id = s_id[slot];
+        const int id = gl[0][slot];   // the slot is the particle's rank in the first run
         const V4<T> q = pos[id];
         tree[t] = {q.x, q.y, q.z, node_word(T(0), id | (a << 24))};
         if (heap_order) hash[t - 1] = {(uint32_t)(t - 1), (uint32_t)id};
@@ -301,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_tree_lds(const int* __restrict__ wor
     }
     __syncthreads();
     // flags of the other two lists, and where each node's placed particle sits in them
-    for (int i = tid; i < r1 - r0; i += kBlock) {
+    for (int i = tid; i < r1 - r0; i += kLdsThreads) {
       const int tb = s_tag[s_list[b][cur[b]][i]], tc = s_tag[s_list[c][cur[c]][i]];
       s_fb[i] = tb >= next_level && !(tb & 1);
       s_fc[i] = tc >= next_level && !(tc & 1);
@@ -309,18 +321,18 @@ __global__ __launch_bounds__(kBlock) void k_tree_lds(const int* __restrict__ wor
       if (tc < next_level) s_at_c[tc - t_base] = (uint16_t)i;
     }
     __syncthreads();
-    // exclusive scans of the flags over the runs (chunks of ceil(R / kBlock) per thread)
+    // exclusive scans of the flags over the runs (chunks of ceil(R / kLdsThreads) per thread)
     {
       const int R = r1 - r0;
-      const int chunk = (R + kBlock - 1) / kBlock;
+      const int chunk = (R + kLdsThreads - 1) / kLdsThreads;
       const int i0 = min(R, tid * chunk), i1 = min(R, i0 + chunk);
       int cb = 0, cc = 0;
       for (int i = i0; i < i1; ++i) {
         cb += s_fb[i];
         cc += s_fc[i];
       }
-      int ob = block_excl_scan(cb, s_tot);
-      int oc = block_excl_scan(cc, s_tot);
+      int ob = cb, oc = cc;
+      block_excl_scan2(ob, oc, s_tot);
       for (int i = i0; i < i1; ++i) {
         s_sb[i] = (uint16_t)ob;
         s_sc[i] = (uint16_t)oc;
@@ -330,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_tree_lds(const int* __restrict__ wor
     }
     __syncthreads();
     // regroup the other two lists into the next level's runs
-    for (int i = tid; i < r1 - r0; i += kBlock) {
+    for (int i = tid; i < r1 - r0; i += kLdsThreads) {
       const int g = r0 + i;
       for (int k = 0; k < 2; ++k) {
         const int ax = k ? c : b;
@@ -420,7 +432,7 @@ void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash
   }
   if (m > 0) {
     const int roots = (int)std::min<long>(1l << L_lds, (long)n - (1l << L_lds) + 1);
-    hipLaunchKernelGGL((k_tree_lds<T>), dim3(roots), dim3(kBlock), 0, s, work, tag, n, L_lds, cur[0], cur[1], cur[2],
+    hipLaunchKernelGGL((k_tree_lds<T>), dim3(roots), dim3(kLdsThreads), 0, s, work, tag, n, L_lds, cur[0], cur[1], cur[2],
                        pos, tree, hash, heap_order);
     CS_CHECK_LAUNCH();
   }
