@@ -186,10 +186,11 @@ def test_mixed_gravity_forms_agree(cs):
     assert not np.array_equal(out["expanded"], out["tile"]), "the expanded body was not taken"
 
 
-@pytest.mark.parametrize("n", [100, 300, 1000])
+@pytest.mark.parametrize("n", [2, 5, 33, 100, 300, 1000])
 def test_mixed_form_small_sizes(cs, orc, n):
     """Small N: most of the 8 j-parts of the gravity grid are empty, sub-tiles
-    are partial and padding is a large share of the records."""
+    are partial and padding is a large share of the records (down to two
+    particles among 256 records, where every walk takes the whole tree)."""
     p = cs.preset_params(2, n)
     pos, vel, hyd = cs.generate_ic(2, n, 3)
     e = cs.make_engine("mixed", 2, n, 3, ph_sort=True)

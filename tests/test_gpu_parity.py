@@ -82,7 +82,7 @@ def test_plain_passes_bit_exact(cs, orc, cfg, n):
             assert e.scalar("dt") == o.scalar("dt"), (e.scalar("dt"), o.scalar("dt"))
 
 
-@pytest.mark.parametrize("cfg,n", CASES + [(2, 1000), (3, 4099), (1, 257)])
+@pytest.mark.parametrize("cfg,n", CASES + [(2, 1000), (3, 4099), (1, 257), (1, 1), (2, 2), (3, 5), (1, 33)])
 def test_plain_steps_bit_exact(cs, orc, cfg, n):
     """Whole steps (tree, h, lists, density, EOS, hydro, reaction, gravity,
     integration, timestep) stay bit-identical over several steps."""
