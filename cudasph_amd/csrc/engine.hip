@@ -85,9 +85,10 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
   CS_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
   CS_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
   cap_ = 100L * nd_;  // ngb_tot_max (read_state.cpp:29)
-  // the plain form keeps the reference-exact host build (its lists follow
-  // the reference's tie-breaking); the others build on the device
-  tree_gpu_ = form != SPH_FORM_PLAIN;
+  // the plain form's lists follow the reference's tie-breaking: its device
+  // build is checked for ties and falls back to the host build when one
+  // decides a node; the other forms build on the device
+  tree_gpu_ = form != SPH_FORM_PLAIN ? 1 : 2;
   // the plain form walks exactly as the reference (its walk lengths are an
   // output, bodies_hash.key); the others prune by subtree boxes
   walk_boxes_ = form != SPH_FORM_PLAIN;
@@ -95,6 +96,7 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
   for (auto* b : {&pos_, &vel_, &hydro_, &dots_, &bdots_, &hydro2_, &grav_, &gpos_, &gvel_, &old_vu_, &tmp4_})
     b->alloc(nd_);
   tree_.alloc(nd_ + 2);
+  ties_.alloc(1);
   boxes_.alloc(2 * ((size_t)nd_ + 2));
   if (!kF32 && gravity64_parts(nd_) > 1) gpart64_.alloc((size_t)gravity64_parts(nd_) * nd_);
   hash_.alloc(nd_);
@@ -269,6 +271,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "walk_boxes") return walk_boxes_;
   if (name == "overlap") return overlap_;
   if (name == "tree_gpu") return tree_gpu_;
+  if (name == "tree_fallbacks") return (double)tree_fallbacks_;
   if (name == "timing") return timers_.enabled;
   if (name == "cap") return (double)cap_;
   if (name == "real_bytes") return (double)sizeof(T);
@@ -313,8 +316,10 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
   } else if (name == "walk_boxes") {
     walk_boxes_ = v != 0;
   } else if (name == "overlap") {
+    if (v != 0 && v != 1) throw std::invalid_argument("overlap must be 0 or 1");
     overlap_ = (int)v;
   } else if (name == "tree_gpu") {
+    if (v != 0 && v != 1 && v != 2) throw std::invalid_argument("tree_gpu must be 0, 1 or 2");
     tree_gpu_ = (int)v;
   } else if (name == "timing") {
     sync();
@@ -385,16 +390,30 @@ void Engine<T>::statistics(double* o) {
 // download; tree_build waits for it, builds the tree and enqueues the upload,
 // so GPU work queued in between (gravity under Plummer softening) runs while
 // the host builds.  Device build (tree_gpu 1): tree_fetch enqueues it, ahead
-// of that GPU work, and tree_build only collects dt.
+// of that GPU work, and tree_build only collects dt.  Verified device build
+// (tree_gpu 2, the plain form's default): the device build, which is the
+// reference's tree unless some node's particle ties its split coordinate
+// with another particle of its subtree (then the reference's quicksort may
+// have picked the other one); the build flags that case, and the host build
+// runs for that step instead, from positions copied beside the device build.
 template <class T>
 void Engine<T>::tree_fetch() {
   if (tree_gpu_) {
     const long need = tree_work_ints(n_);
     if ((long)tree_work_.n < need) tree_work_.alloc(need);
     TimerScope ts(timers_, "tree_build", s_);
+    int* ties = nullptr;
+    if (tree_gpu_ == 2) {
+      CS_HIP(hipMemsetAsync(ties_, 0, sizeof(int), s_));
+      ties = ties_.p;
+    }
     launch_tree_build<T>(pos_, n_, nd_, tree_, hash_, i_order_, tree_work_, pairs_, pairs_alt_, sort_tmp_, scan_tmp_,
-                         s_);
+                         s_, ties);
     boxes_stale_ = true;
+    if (tree_gpu_ == 2) {
+      CS_HIP(hipMemcpyAsync(h_pos_.p, pos_, (size_t)nd_ * sizeof(V4<T>), hipMemcpyDeviceToHost, s_));
+      CS_HIP(hipMemcpyAsync(&h_small_[5], ties_, sizeof(int), hipMemcpyDeviceToHost, s_));
+    }
   } else {
     CS_HIP(hipMemcpyAsync(h_pos_.p, pos_, (size_t)nd_ * sizeof(V4<T>), hipMemcpyDeviceToHost, s_));
   }
@@ -406,7 +425,13 @@ template <class T>
 void Engine<T>::tree_build() {
   CS_HIP(hipEventSynchronize(fetched_));
   std::memcpy(&dt_, &h_small_[0], 8);
-  if (tree_gpu_) return;
+  if (tree_gpu_ == 1) return;
+  if (tree_gpu_ == 2) {
+    int tied = 0;
+    std::memcpy(&tied, &h_small_[5], sizeof(int));
+    if (!tied) return;
+    ++tree_fallbacks_;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   build_kd_tree<T>(h_pos_.p, n_, nd_, p_.n_threads, h_tree_.p, h_hash_.p);
   if (i_order_ == 0)   // lanes take particles in memory order (the reference's identity hash)
@@ -797,13 +822,13 @@ void Engine<T>::step() {
   // (sph_accelerations, then CUDA_gravity, step.cpp:65,94).  The spline
   // softening needs the new ranges and keeps that order.
   const bool grav_first = p_.grav_mode == 1;
-  // overlap < 0 (the default): on for the mixed form, whose SPH passes take
-  // ~12 ms at N = 2^20 and slow its gravity by ~4 ms when they run beside it
-  // (288.5 -> 282.9 ms/step; with the far field 106.2 -> 101.6), and for the
-  // FP64 form (827 -> 814 ms/step at 2^20, 58.2 -> 53.4 at 262 144, 6.80 ->
-  // 6.70 at 65 536); off for the plain form, whose host tree build and
-  // unpruned h-solve walks lose more than they hide (1057 -> 1215 ms/step)
-  const bool overlap = overlap_ >= 0 ? overlap_ != 0 : form_ != SPH_FORM_PLAIN;
+  // overlap < 0 (the default) is on: the mixed form's SPH passes take ~12 ms
+  // at N = 2^20 and slow its gravity by ~4 ms when they run beside it
+  // (288.5 -> 282.9 ms/step; with the far field 106.2 -> 101.6); FP64 827 ->
+  // 814 ms/step at 2^20, 58.2 -> 53.4 at 262 144; plain, with its checked
+  // device tree build, 3.63 -> 2.22 ms/step at 16 384, 12.8 -> 8.5 at 65 536,
+  // 81.5 -> 71.3 at 262 144, 1056 -> 1036 at 2^20
+  const bool overlap = overlap_ != 0;
   // on its own stream, gravity starts before the device tree build, which
   // then runs beside it (the build is on the SPH passes' path, not gravity's)
   const bool grav_now = grav_first && overlap && tree_gpu_;

@@ -204,9 +204,9 @@ class Engine final : public EngineBase {
 
   double dt_ = 0, global_time_ = 0;
   int iteration_ = 0, exact_rsqrt_ = 0;
-  // 1: gravity on gs_ beside the SPH passes (Plummer softening), 0: in
-  // order, -1 (default): in the mixed form only (measured in Engine::step)
-  int overlap_ = -1;
+  // 1 (default): gravity on gs_ beside the SPH passes (Plummer softening),
+  // 0: in order (measured in Engine::step)
+  int overlap_ = 1;
   int grav_split_only_ = 0;
   int grav_unit_mass_ = 1;    // mixed gravity: unit-mass pair body when every mass is equal
   int grav_expand_ = 1;       // mixed gravity: expanded r^2 in the equal-mass tile body
@@ -218,7 +218,9 @@ class Engine final : public EngineBase {
   // 1: the kd-tree is built on the device (same tree without coordinate
   // ties; with ties the same neighbour sets; the mixed form's default);
   // 0: on the host, bit-identical to the reference's quicksort build
-  int tree_gpu_ = 0;
+  int tree_gpu_ = 0;            // 0 host build, 1 device build, 2 device build checked for ties (host on a tie)
+  long tree_fallbacks_ = 0;     // steps whose verified device build fell back to the host build
+  DBuf<int> ties_;
   int walk_boxes_ = 0;   // walks prune subtrees by their bounding boxes (mixed and FP64 forms)
   DBuf<int> tree_work_;   // allocated on first use of the device build
   long ngb_tot_ = 0;

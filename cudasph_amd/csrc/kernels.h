@@ -116,9 +116,14 @@ void launch_ph_keys(const V4<T>* pos, int n, int bits, uint32_t* key_hi, uint32_
 // tree_work_ints(n) ints; kv and kv_alt 3 n records each; sort_scratch
 // 3 radix_scratch_ints(n) and scan_scratch scan_scratch_ints(n) ints.
 long tree_work_ints(long n);
+// ties (may be null): set to 1 when some node's particle shares its split
+// coordinate with another particle of its subtree -- the one case in which
+// the reference's quicksort may have picked the other particle as the node;
+// otherwise the tree is the reference's, bit for bit.
 template <class T>
 void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash, int heap_order, int* work,
-                       KV* kv, KV* kv_alt, int* sort_scratch, int* scan_scratch, hipStream_t s);
+                       KV* kv, KV* kv_alt, int* sort_scratch, int* scan_scratch, hipStream_t s,
+                       int* ties = nullptr);
 // Subtree bounding boxes of a kd-tree over n_nodes particles: box[2k] (low
 // corner) and box[2k+1] (high corner) of node k's subtree, for k up to
 // n_nodes+1; FP32, rounded outward.  One launch per tree level.

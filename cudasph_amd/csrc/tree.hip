@@ -131,7 +131,8 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_tree_split(const int* __restrict__ la, int* __restrict__ la_next,
                                                         int* __restrict__ tag, const V4<T>* __restrict__ pos,
                                                         V4<T>* __restrict__ tree, KV* __restrict__ hash, int m,
-                                                        int L, int n, int axis, int heap_order) {
+                                                        int L, int n, int axis, int heap_order,
+                                                        int* __restrict__ ties) {
   const int idx = blockIdx.x * kBlock + threadIdx.x;
   if (idx >= m) return;
   const int id = la[idx];
@@ -146,6 +147,14 @@ __global__ __launch_bounds__(kBlock) void k_tree_split(const int* __restrict__ l
     const V4<T> q = pos[id];
     tree[t] = {q.x, q.y, q.z, node_word(T(0), id | (axis << 24))};
     if (heap_order) hash[t - 1] = {(uint32_t)(t - 1), (uint32_t)id};
+    // a run neighbour with the node's coordinate: which of the two becomes
+    // the node is the presort's choice, not necessarily the reference's
+    if (ties) {
+      const T c = coord(q, axis);
+      if ((p > 0 && coord(pos[la[idx - 1]], axis) == c) ||
+          (p + 1 < subtree_size(t, n) && coord(pos[la[idx + 1]], axis) == c))
+        atomicOr(ties, 1);
+    }
   } else {
     tag[id] = 2 * t + 1;
     la_next[idx - placed_before - 1] = id;
@@ -249,7 +258,7 @@ template <class T>
 __global__ __launch_bounds__(kLdsThreads) void k_tree_lds(const int* __restrict__ work, int* __restrict__ slot_of, int n,
                                                       int L0, int cur0, int cur1, int cur2,
                                                       const V4<T>* __restrict__ pos, V4<T>* __restrict__ tree,
-                                                      KV* __restrict__ hash, int heap_order) {
+                                                      KV* __restrict__ hash, int heap_order, int* __restrict__ ties) {
   __shared__ uint16_t s_list[3][2][kLdsMax];   // slots, per axis, double-buffered
   __shared__ int s_tag[kLdsMax];               // the heap node each slot's particle belongs to
   __shared__ uint8_t s_fb[kLdsMax], s_fc[kLdsMax];
@@ -306,6 +315,12 @@ __global__ __launch_bounds__(kLdsThreads) void k_tree_lds(const int* __restrict_
         const V4<T> q = pos[id];
         tree[t] = {q.x, q.y, q.z, node_word(T(0), id | (a << 24))};
         if (heap_order) hash[t - 1] = {(uint32_t)(t - 1), (uint32_t)id};
+        if (ties) {   // as in k_tree_split
+          const T cq = coord(q, a);
+          if ((p > 0 && coord(pos[gl[0][s_list[a][cur[a]][i - 1]]], a) == cq) ||
+              (p + 1 < subtree_size(t, n) && coord(pos[gl[0][s_list[a][cur[a]][i + 1]]], a) == cq))
+            atomicOr(ties, 1);
+        }
       } else {
         s_tag[slot] = 2 * t + 1;
         s_list[a][cur[a] ^ 1][g - pb - 1 - r0n] = (uint16_t)slot;
@@ -375,7 +390,7 @@ long tree_work_ints(long n) { return 13 * n + 8; }
 
 template <class T>
 void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash, int heap_order, int* work,
-                       KV* kv, KV* kv_alt, int* sort_scratch, int* scan_scratch, hipStream_t s) {
+                       KV* kv, KV* kv_alt, int* sort_scratch, int* scan_scratch, hipStream_t s, int* ties) {
   if (n <= 0 || n > n_dev) throw std::invalid_argument("tree build: need 0 < n <= n_dev");
   hipLaunchKernelGGL((k_tree_fixed<T>), dim3(ceil_div(n_dev + 2, kBlock)), dim3(kBlock), 0, s, tree, hash, n, n_dev,
                      heap_order);
@@ -414,7 +429,7 @@ void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash
     const int a = L % 3, b = (L + 1) % 3, c = (L + 2) % 3;
     const dim3 gm(ceil_div(m, kBlock));
     hipLaunchKernelGGL((k_tree_split<T>), gm, dim3(kBlock), 0, s, list[a][cur[a]], list[a][cur[a] ^ 1], tag, pos,
-                       tree, hash, m, L, n, a, heap_order);
+                       tree, hash, m, L, n, a, heap_order, ties);
     CS_CHECK_LAUNCH();
     const int placed = (int)std::min<long>(1l << L, (long)n - ((1l << L) - 1));
     if (m - placed > 0) {
@@ -433,7 +448,7 @@ void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash
   if (m > 0) {
     const int roots = (int)std::min<long>(1l << L_lds, (long)n - (1l << L_lds) + 1);
     hipLaunchKernelGGL((k_tree_lds<T>), dim3(roots), dim3(kLdsThreads), 0, s, work, tag, n, L_lds, cur[0], cur[1], cur[2],
-                       pos, tree, hash, heap_order);
+                       pos, tree, hash, heap_order, ties);
     CS_CHECK_LAUNCH();
   }
 }
@@ -502,8 +517,10 @@ void launch_tree_boxes(const V4<T>* tree, F4* box, int n_nodes, int fix_phantom,
   }
 }
 
-template void launch_tree_build<float>(const F4*, int, int, F4*, KV*, int, int*, KV*, KV*, int*, int*, hipStream_t);
-template void launch_tree_build<double>(const D4*, int, int, D4*, KV*, int, int*, KV*, KV*, int*, int*, hipStream_t);
+template void launch_tree_build<float>(const F4*, int, int, F4*, KV*, int, int*, KV*, KV*, int*, int*, hipStream_t,
+                                       int*);
+template void launch_tree_build<double>(const D4*, int, int, D4*, KV*, int, int*, KV*, KV*, int*, int*, hipStream_t,
+                                        int*);
 template void launch_tree_boxes<float>(const F4*, F4*, int, int, hipStream_t);
 template void launch_tree_boxes<double>(const D4*, F4*, int, int, hipStream_t);
 

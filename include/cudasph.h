@@ -192,14 +192,18 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
 /* Scalars.  Read and write: dt, global_time, iteration; exact_rsqrt (plain
  * form: correctly rounded 1/sqrt in gravity, the CPU restatement's bits; 0
  * is the hardware rsq the reference uses); i_order (per-particle kernels in
- * memory order 0 or kd-tree heap order 1); overlap (1: gravity on a second
- * stream beside the SPH passes, 0: in order, -1, the default: in the mixed
- * and FP64 forms; the results are the same bits either way);
+ * memory order 0 or kd-tree heap order 1); overlap (1, the default: gravity
+ * on a second stream beside the SPH passes, 0: in order; the results are the
+ * same bits either way);
  * timing (per-kernel HIP-event timers);
  * tree_gpu (1: the kd-tree built on the device, the mixed and FP64 forms'
  * default -- the same tree as the host build when no two particles share a
  * coordinate, otherwise the same neighbour sets; 0: the host build,
- * bit-identical to kd_tree.cpp:39-128, the plain form's default);
+ * bit-identical to kd_tree.cpp:39-128; 2, the plain form's default: the
+ * device build, checked at every node for a particle that shares the node's
+ * split coordinate, the only case where the two builds can differ, and the
+ * host build for that step when one does -- the reference's tree either way;
+ * read-only tree_fallbacks counts those steps);
  * walk_boxes (1, the mixed and FP64 forms' default: the tree walks also skip
  * a child whose subtree bounding box lies beyond the range -- the same
  * neighbours in the same order, so the same results, with fewer nodes

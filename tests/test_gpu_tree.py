@@ -135,3 +135,27 @@ def test_mixed_steps_with_device_tree(cs):
     assert abs(s1["E"] - s0["E"]) <= 1e-6 * abs(s0["E"])
     assert np.abs(p1[:, :3] - p0[:, :3]).max() < 1e-6
     assert np.allclose(g1, g0, rtol=1e-5, atol=1e-6 * np.abs(g0).max())
+
+
+@pytest.mark.parametrize("n,grid", [(16384, 0), (50000, 0), (16384, 32), (30000, 64)])
+def test_checked_device_tree_is_the_host_tree(cs, n, grid):
+    """tree_gpu = 2 (the plain form's default): the device build, flagged when
+    a node's particle ties its split coordinate within its subtree, with the
+    host build for that pass when flagged -- the host tree in every bit,
+    ties or not.  grid 0 gives distinct coordinates (no fallback); grid > 0
+    snaps them to a grid, so the build must fall back."""
+    print(f"checked tree n={n} grid={grid}", flush=True)
+    e, p = engine(cs, "plain", 1, n)
+    if grid:
+        xyz = np.round(distinct_xyz(n, seed=n) * grid) / grid
+    else:
+        xyz = distinct_xyz(n, seed=n)
+    set_xyz(e, n, xyz)
+    th, hh = tree_and_hash(cs, e, 0)
+    before = e.scalar("tree_fallbacks")
+    tg, hg = tree_and_hash(cs, e, 2)
+    fell_back = e.scalar("tree_fallbacks") - before
+    assert fell_back == (1 if grid else 0), fell_back
+    assert_bits(tg, th, "tree")
+    assert np.array_equal(hg, hh), "particle map differs"
+    e.close()
