@@ -262,6 +262,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_split_only") return grav_split_only_;
   if (name == "grav_unit_mass") return grav_unit_mass_;
   if (name == "grav_expand") return grav_expand_;
+  if (name == "grav_sums") return grav_sums_;
   if (name == "adj_free_ranges") return adj_free_ranges_;
   if (name == "adj_implicit_h") return adj_implicit_h_;
   if (name == "grav_far") return grav_far_;
@@ -299,6 +300,9 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     grav_unit_mass_ = (int)v;
   } else if (name == "grav_expand") {
     grav_expand_ = (int)v;
+  } else if (name == "grav_sums") {
+    if (v != -1 && v != 0 && v != 1) throw std::invalid_argument("grav_sums must be -1, 0 or 1");
+    grav_sums_ = (int)v;
   } else if (name == "adj_free_ranges") {
     adj_free_ranges_ = (int)v;
   } else if (name == "adj_implicit_h") {
@@ -685,7 +689,7 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
       return;
     }
   }
-  launch_gravity<T>(gpos_, gvel_, grav_, nd_, nd_, p_.grav_mode, T(p_.grav_eps), exact_rsqrt_, gs);
+  launch_gravity<T>(gpos_, gvel_, grav_, nd_, nd_, p_.grav_mode, T(p_.grav_eps), exact_rsqrt_, gs, grav_sums_);
 }
 
 template <class T>

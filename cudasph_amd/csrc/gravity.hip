@@ -214,6 +214,116 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity(const V4<T>* __restrict_
   if (i < n_i) out[i] = {a[0].get(), a[1].get(), a[2].get(), a[3].get()};
 }
 
+// Plain form, Plummer softening (the presets' softening).  A particle's four
+// per-pair dreal sums (ax, ay, az, potential) are serial chains from record
+// to record, and the issue is in order, so both kernels below are
+// software-pipelined: the terms of the next G records are computed in the
+// same block as the chain over the current G, which gives the issue
+// independent work between the chain's dependent steps.  The terms enter the
+// chains in ascending j and each term is pair_plummer's expression, so the
+// sums are pair_plummer's bits.
+template <int C, bool EXACT>
+__device__ __forceinline__ float plummer_term(const F4& pj, const F4& pi, float eps2) {
+  const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+  const float d2 = rx * rx + ry * ry + rz * rz + eps2;
+  const float inv = inv_sqrt<EXACT>(d2);
+  if constexpr (C == 3) {
+    return pj.w * inv;
+  } else {
+    const float s3 = pj.w * (inv * inv * inv);
+    return (C == 0 ? rx : C == 1 ? ry : rz) * s3;
+  }
+}
+
+template <int G, class Term, class Chain>
+__device__ __forceinline__ void pipelined(int cnt, Term term, Chain chain) {
+  using Y = decltype(term(0));
+  int j = 0;
+  if (cnt >= G) {
+    Y y[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) y[k] = term(k);
+    for (j = G; j + G <= cnt; j += G) {
+      Y yn[G];
+#pragma unroll
+      for (int k = 0; k < G; ++k) yn[k] = term(j + k);
+#pragma unroll
+      for (int k = 0; k < G; ++k) chain(y[k]);
+#pragma unroll
+      for (int k = 0; k < G; ++k) y[k] = yn[k];
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) chain(y[k]);   // records j - G .. j - 1
+  }
+  for (; j < cnt; ++j) chain(term(j));
+}
+
+// One lane per particle, its four chains side by side, one record ahead
+// (measured against the same loop without the pipelining: 8.66 -> 7.38 ms at
+// N = 65 536, 71.5 -> 66.9 ms at 262 144, 1037 -> 1012 ms at 2^20; two
+// records ahead gained less).
+template <bool EXACT>
+__global__ __launch_bounds__(kGravBlock) void k_gravity_lane(const F4* __restrict__ gpos, F4* __restrict__ out,
+                                                             int n_i, int n_j, float eps2) {
+  __shared__ F4 s_pos[kJTile];
+  const int i = blockIdx.x * kGravBlock + threadIdx.x;
+  const F4 pi = gpos[i < n_i ? i : n_i - 1];
+  DR a0{0.f, 0.f}, a1{0.f, 0.f}, a2{0.f, 0.f}, a3{0.f, 0.f};
+  for (int t0 = 0; t0 < n_j; t0 += kJTile) {
+    const int cnt = stage_tile(s_pos, gpos, t0, n_j);
+    pipelined<1>(
+        cnt,
+        [&](int j) {
+          const F4 pj = s_pos[j];
+          return F4{plummer_term<0, EXACT>(pj, pi, eps2), plummer_term<1, EXACT>(pj, pi, eps2),
+                    plummer_term<2, EXACT>(pj, pi, eps2), plummer_term<3, EXACT>(pj, pi, eps2)};
+        },
+        [&](const F4& y) {
+          dr_sub(a0, y.x);
+          dr_sub(a1, y.y);
+          dr_sub(a2, y.z);
+          dr_sub(a3, y.w);
+        });
+  }
+  if (i < n_i) out[i] = {a0.hi, a1.hi, a2.hi, a3.hi};
+}
+
+// Few particles: one wavefront per sum.  A block takes 64 particles and its
+// wavefront c keeps sum c of each, repeating the pair's distance and rsq and
+// running one two-sum per pair instead of four (~23 VALU + rsq per pair and
+// lane against ~58 + rsq), on four times as many wavefronts: at N = 16 384
+// the one-lane kernel has a wavefront for 256 of the GPU's 1024 SIMDs, this
+// one fills all of them, 8 records ahead (16 384: 2.19 -> 0.89 ms; 49 152:
+// 6.55 -> 5.16 ms; 65 536: 8.9 ms against the one-lane kernel's 7.4).
+constexpr int kSumParticles = kGravBlock / 4;   // particles per block of k_gravity_sums
+
+template <int C, bool EXACT>
+__device__ __forceinline__ void plummer_sum(DR& acc, const F4* __restrict__ s_pos, int cnt, const F4& pi,
+                                            float eps2) {
+  pipelined<8>(
+      cnt, [&](int j) { return plummer_term<C, EXACT>(s_pos[j], pi, eps2); }, [&](float y) { dr_sub(acc, y); });
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(kGravBlock) void k_gravity_sums(const F4* __restrict__ gpos, F4* __restrict__ out,
+                                                             int n_i, int n_j, float eps2) {
+  __shared__ F4 s_pos[kJTile];
+  const int c = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);   // the wavefront's sum
+  const int i = blockIdx.x * kSumParticles + (threadIdx.x & 63);
+  const F4 pi = gpos[i < n_i ? i : n_i - 1];
+  DR acc{0.f, 0.f};
+  for (int t0 = 0; t0 < n_j; t0 += kJTile) {
+    const int cnt = stage_tile(s_pos, gpos, t0, n_j);
+    switch (c) {
+      case 0: plummer_sum<0, EXACT>(acc, s_pos, cnt, pi, eps2); break;
+      case 1: plummer_sum<1, EXACT>(acc, s_pos, cnt, pi, eps2); break;
+      case 2: plummer_sum<2, EXACT>(acc, s_pos, cnt, pi, eps2); break;
+      default: plummer_sum<3, EXACT>(acc, s_pos, cnt, pi, eps2); break;
+    }
+  }
+  if (i < n_i) reinterpret_cast<float*>(out + i)[c] = acc.hi;
+}
+
 // ---------------------------------------------------------------------------
 // Multipole far field of the mixed Plummer kernel (optional, flags bit 4).
 //
@@ -999,18 +1109,28 @@ void launch_gravity64_plummer(const D4* gpos, D4* out, D4* part, int n_i, int n_
   }
 }
 
+bool gravity_sums_apart(int n_i, int sums) {
+  return sums >= 0 ? sums != 0 : n_i <= kSumsApartMaxN;
+}
+
 template <class T>
 void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, int n_j, int soft, T eps,
-                    int exact, hipStream_t s) {
+                    int exact, hipStream_t s, int sums) {
   const dim3 grid(ceil_div(n_i, kGravBlock)), block(kGravBlock);
   const T eps2 = eps * eps;
   if constexpr (sizeof(T) == 4) {
-    if (soft == 0) {
+    if (soft == 1) {
+      if (gravity_sums_apart(n_i, sums)) {
+        const dim3 grid4(ceil_div(n_i, kSumParticles));
+        if (exact) hipLaunchKernelGGL(k_gravity_sums<true>, grid4, block, 0, s, gpos, out, n_i, n_j, eps2);
+        else hipLaunchKernelGGL(k_gravity_sums<false>, grid4, block, 0, s, gpos, out, n_i, n_j, eps2);
+      } else {
+        if (exact) hipLaunchKernelGGL(k_gravity_lane<true>, grid, block, 0, s, gpos, out, n_i, n_j, eps2);
+        else hipLaunchKernelGGL(k_gravity_lane<false>, grid, block, 0, s, gpos, out, n_i, n_j, eps2);
+      }
+    } else {
       if (exact) hipLaunchKernelGGL((k_gravity<float, 0, 0, true>), grid, block, 0, s, gpos, gvel, out, n_i, n_j, eps2);
       else hipLaunchKernelGGL((k_gravity<float, 0, 0, false>), grid, block, 0, s, gpos, gvel, out, n_i, n_j, eps2);
-    } else {
-      if (exact) hipLaunchKernelGGL((k_gravity<float, 1, 0, true>), grid, block, 0, s, gpos, gvel, out, n_i, n_j, eps2);
-      else hipLaunchKernelGGL((k_gravity<float, 1, 0, false>), grid, block, 0, s, gpos, gvel, out, n_i, n_j, eps2);
     }
   } else {
     if (soft == 0) hipLaunchKernelGGL((k_gravity<double, 0, 2, true>), grid, block, 0, s, gpos, gvel, out, n_i, n_j, eps2);
@@ -1085,7 +1205,7 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
   CS_CHECK_LAUNCH();
 }
 
-template void launch_gravity<float>(const F4*, const F4*, F4*, int, int, int, float, int, hipStream_t);
-template void launch_gravity<double>(const D4*, const D4*, D4*, int, int, int, double, int, hipStream_t);
+template void launch_gravity<float>(const F4*, const F4*, F4*, int, int, int, float, int, hipStream_t, int);
+template void launch_gravity<double>(const D4*, const D4*, D4*, int, int, int, double, int, hipStream_t, int);
 
 }  // namespace cudasph

@@ -49,9 +49,14 @@ void launch_hydro(const V4<T>* pos, const V4<T>* vel, const V4<T>* hydro, const 
                   int n_dev, T alpha, T beta, hipStream_t s);
 
 // ---- gravity.hip ----
+// sums: for the FP32 Plummer kernel, 1 keeps each particle's four sums in
+// four wavefronts (k_gravity_sums), 0 in one lane, -1 (auto) four
+// wavefronts up to kSumsApartMaxN particles; the same bits either way
+constexpr int kSumsApartMaxN = 49152;
+bool gravity_sums_apart(int n_i, int sums);
 template <class T>
 void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, int n_j, int soft, T eps,
-                    int exact, hipStream_t s);
+                    int exact, hipStream_t s, int sums = -1);
 // FP64 form, Plummer softening (fused pair body; the j-range split into
 // gravity64_parts(n_i) parts when the i-blocks alone would not fill the GPU,
 // `part` then holding parts * n_i records)

@@ -213,6 +213,9 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
  * tolerance: grav_split_only (1: split hi/lo differences for every pair),
  * grav_unit_mass (0: the general-mass body even when every mass is equal),
  * grav_expand (0: the equal-mass tile body on r instead of the expanded one).
+ * Plain-form Plummer gravity, the same bits either way: grav_sums (1: each
+ * particle's four sums in four wavefronts, 0: in one lane, -1, the default:
+ * four wavefronts up to 49 152 particles).
  * Mixed-form gravity options: grav_far (1: an order-4 multipole far field for
  * each cell of 16384, 4096, 1024 or 256 records, or 64-record sub-tile, that a
  * whole wavefront sees from beyond grav_far_ratio of its radii, default 8,

@@ -450,3 +450,24 @@ def test_mixed_state_file_keeps_fp64_state(cs, tmp_path):
     e.close()
     e2.close()
 
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 4099, 16384, 49152, 49153])
+@pytest.mark.parametrize("exact", [0, 1])
+def test_plain_gravity_sums_apart_change_nothing(cs, n, exact):
+    """The plain form's Plummer gravity with each particle's four sums in
+    four wavefronts gives the bits of the one-lane kernel (hardware and
+    correctly rounded rsqrt), at sizes around the 64-particle blocks and on
+    either side of the automatic switch at 49 152."""
+    print(f"plain gravity sums n={n} exact={exact}", flush=True)
+    out = []
+    for sums in (0, 1):
+        e = cs.make_engine("plain", 1, n, 4, ph_sort=True)
+        assert e.params.grav_mode == 1
+        e.set_scalar("exact_rsqrt", exact)
+        e.set_scalar("grav_sums", sums)
+        e.init()
+        e.run_pass(cs.GRAVITY)
+        out.append(e.get("grav")[:n].copy())
+        e.close()
+    assert_bits(out[1], out[0], "grav")
