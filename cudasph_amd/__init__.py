@@ -214,7 +214,7 @@ class Engine:
             "ngb_count": (np.int32, (nd,)), "ngb_offset": (np.int32, (nd + 1,)),
             "ngb_list": (np.int32, (cap,)), "ngb_acc": (r, (cap,)),
             "pos64": (np.float64, (nd, 4)), "vel64": (np.float64, (nd, 4)), "u64": (np.float64, (nd,)),
-            "pos_lo": (np.float32, (nd, 4)),
+            "pos_lo": (np.float32, (nd, 4)), "boxes": (np.float32, (2 * (nd + 2), 4)),
         }
         return table[name]
 

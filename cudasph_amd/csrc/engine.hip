@@ -223,6 +223,7 @@ long Engine<T>::get(const std::string& name, void* host, long mb) {
   if (name == "hydro2") return take(hydro2_, hydro2_.bytes());
   if (name == "grav") return take(grav_, grav_.bytes());
   if (name == "tree") return take(tree_, tree_.bytes());
+  if (name == "boxes") return take(boxes_, boxes_.bytes());
   if (name == "hash") return take(hash_, hash_.bytes());
   if (name == "n_ngb") return take(n_ngb_, n_ngb_.bytes());
   if (name == "ngb_count") return take(ngb_count_, ngb_count_.bytes());

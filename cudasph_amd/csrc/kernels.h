@@ -131,7 +131,8 @@ void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash
                        int* ties = nullptr);
 // Subtree bounding boxes of a kd-tree over n_nodes particles: box[2k] (low
 // corner) and box[2k+1] (high corner) of node k's subtree, for k up to
-// n_nodes+1; FP32, rounded outward.  One launch per tree level.
+// n_nodes+1; FP32, rounded outward.  Two launches: one block per subtree
+// below level 10, then one block for the levels above.
 template <class T>
 void launch_tree_boxes(const V4<T>* tree, F4* box, int n_nodes, int fix_phantom, hipStream_t s);
 

@@ -478,13 +478,11 @@ __device__ __forceinline__ float hi_bound(double x) {
   return (double)f < x ? float_above(f) : f;
 }
 
-// boxes of the nodes first..last (one tree level): the node's own particle
-// and the boxes of the children a walk can enter from it
+// box of node k: the node's own particle and the boxes of the children a
+// walk can enter from it (their boxes written before)
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_tree_boxes(const V4<T>* __restrict__ tree, F4* __restrict__ box,
-                                                        int n_nodes, int fix_phantom, int first, int last) {
-  const int k = first + blockIdx.x * kBlock + threadIdx.x;
-  if (k > last) return;
+__device__ __forceinline__ void box_node(const V4<T>* __restrict__ tree, F4* box, int n_nodes, int fix_phantom,
+                                         int k) {
   const V4<T> nd = tree[k];
   F4 lo = {lo_bound(nd.x), lo_bound(nd.y), lo_bound(nd.z), 0.f};
   F4 hi = {hi_bound(nd.x), hi_bound(nd.y), hi_bound(nd.z), 0.f};
@@ -501,18 +499,49 @@ __global__ __launch_bounds__(kBlock) void k_tree_boxes(const V4<T>* __restrict__
   box[2 * k + 1] = hi;
 }
 
+// One block per subtree rooted at level S (node 2^S + blockIdx.x): its levels
+// deepest first, with a barrier between levels, so that a node's children
+// were written by the same block before it reads them.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_tree_boxes_sub(const V4<T>* __restrict__ tree, F4* box, int n_nodes,
+                                                            int fix_phantom, int S, int levels, int last_node) {
+  const long r = (1l << S) + blockIdx.x;
+  for (int L = levels; L >= S; --L) {
+    const long first = r << (L - S), cnt = 1l << (L - S);
+    for (long q = threadIdx.x; q < cnt; q += kBlock)
+      if (first + q <= last_node) box_node<T>(tree, box, n_nodes, fix_phantom, (int)(first + q));
+    __syncthreads();
+  }
+}
+
+// the levels above S, in one block
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_tree_boxes_top(const V4<T>* __restrict__ tree, F4* box, int n_nodes,
+                                                            int fix_phantom, int S, int last_node) {
+  for (int L = S - 1; L >= 0; --L) {
+    const int last = (int)std::min<long>((2l << L) - 1, (long)last_node);
+    for (int k = (1 << L) + (int)threadIdx.x; k <= last; k += kBlock) box_node<T>(tree, box, n_nodes, fix_phantom, k);
+    __syncthreads();
+  }
+}
+
 }  // namespace
+
+constexpr int kBoxSubtreeLevel = 10;
 
 template <class T>
 void launch_tree_boxes(const V4<T>* tree, F4* box, int n_nodes, int fix_phantom, hipStream_t s) {
   const int last_node = n_nodes + 1;   // the phantom's box is its record's point
   int levels = 0;
   while ((2l << levels) <= (long)last_node) ++levels;   // deepest level: floor(log2(last_node))
-  for (int L = levels; L >= 0; --L) {
-    const int first = 1 << L;
-    const int last = (int)std::min<long>((2l << L) - 1, (long)last_node);
-    hipLaunchKernelGGL(k_tree_boxes<T>, dim3(ceil_div(last - first + 1, kBlock)), dim3(kBlock), 0, s, tree, box,
-                       n_nodes, fix_phantom, first, last);
+  // subtrees rooted at level S, up to 2^10 of them, then the levels above
+  const int S = std::min(levels, kBoxSubtreeLevel);
+  hipLaunchKernelGGL(k_tree_boxes_sub<T>, dim3(1 << S), dim3(kBlock), 0, s, tree, box, n_nodes, fix_phantom, S, levels,
+                     last_node);
+  CS_CHECK_LAUNCH();
+  if (S > 0) {
+    hipLaunchKernelGGL(k_tree_boxes_top<T>, dim3(1), dim3(kBlock), 0, s, tree, box, n_nodes, fix_phantom, S,
+                       last_node);
     CS_CHECK_LAUNCH();
   }
 }

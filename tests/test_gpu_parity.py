@@ -265,3 +265,19 @@ def test_list_slot_overflow_recount_bit_exact(cs, orc):
     le, lo = lists_of(e, n), lists_of(o, n)
     bad = [i for i in range(n) if not np.array_equal(le[i], lo[i])]
     assert not bad, f"{len(bad)} neighbour lists differ, first particle {bad[0]}"
+
+
+@pytest.mark.parametrize("n", [1, 2, 2047, 2048, 2049, 524288, 524289, 1048577])
+def test_offsets_scan_sizes(cs, n):
+    """The list offsets -- the exclusive scan of n_norm(count, 4), the total
+    last (sph_accelerations.cpp:58-64) -- at sizes around the scan's tiles of
+    2048 values and its partials' chunks of 256 tiles, against numpy."""
+    print(f"offsets scan n={n}", flush=True)
+    e = cs.Engine("plain", cs.preset_params(1, n))
+    cnt = np.random.default_rng(n).integers(0, 60, e.nd).astype(np.int32)
+    e.set("n_ngb", cnt)
+    e.run_pass(cs.OFFSETS)
+    norm = np.maximum(4, (cnt.astype(np.int64) + 3) // 4 * 4)
+    want = np.concatenate([[0], np.cumsum(norm)])
+    assert np.array_equal(e.get("ngb_offset").astype(np.int64), want)
+    e.close()

@@ -159,3 +159,50 @@ def test_checked_device_tree_is_the_host_tree(cs, n, grid):
     assert_bits(tg, th, "tree")
     assert np.array_equal(hg, hh), "particle map differs"
     e.close()
+
+
+def expected_boxes(tree, n, fix_phantom):
+    """Subtree boxes restated in numpy: node k's box bounds its record's point
+    (FP64 coordinates rounded outward to float) and the boxes of the children
+    a walk can enter from it -- 2k and 2k+1 up to n, and node n+1 from 2k = n
+    unless fix_phantom.  Children have larger indices, so one pass from the
+    last node down sees them first."""
+    last = n + 1
+    pts = tree[:last + 1, :3]
+    if pts.dtype == np.float64:
+        f = pts.astype(np.float32)
+        lo = np.where(f.astype(np.float64) > pts, np.nextafter(f, np.float32(-np.inf)), f)
+        hi = np.where(f.astype(np.float64) < pts, np.nextafter(f, np.float32(np.inf)), f)
+    else:
+        lo, hi = pts.copy(), pts.copy()
+    for k in range(last, 0, -1):
+        for ch in (2 * k, 2 * k + 1):
+            if ch > n and not (ch == n + 1 and 2 * k == n and not fix_phantom):
+                continue
+            lo[k] = np.minimum(lo[k], lo[ch])
+            hi[k] = np.maximum(hi[k], hi[ch])
+    return lo, hi
+
+
+@pytest.mark.parametrize("form,n", [("mixed", 1), ("mixed", 2), ("mixed", 7), ("mixed", 1000), ("mixed", 16000),
+                                    ("mixed", 16001), ("mixed", 16384), ("mixed", 70001), ("fp64", 5),
+                                    ("fp64", 16384)])
+@pytest.mark.parametrize("fix_phantom", [0, 1])
+def test_subtree_boxes(cs, form, n, fix_phantom):
+    """The walks' subtree boxes (one block per subtree below level 10, one
+    block above) against the numpy restatement, every node up to n+1."""
+    print(f"boxes form={form} n={n} fix_phantom={fix_phantom}", flush=True)
+    cfg = 3 if n >= 5 else 1
+    p = cs.preset_params(cfg, n)
+    p.fix_phantom = fix_phantom
+    e = cs.Engine(form, p)
+    pos, vel, hyd = cs.generate_ic(cfg, n, 11)
+    e.load_ic(pos, vel, hyd)
+    e.run_pass(cs.TREE)
+    e.run_pass(cs.SOLVE)
+    tree, box = e.get("tree"), e.get("boxes")
+    lo, hi = expected_boxes(tree, n, fix_phantom)
+    k = np.arange(1, n + 2)
+    assert np.array_equal(box[2 * k, :3], lo[k]), "low corners differ"
+    assert np.array_equal(box[2 * k + 1, :3], hi[k]), "high corners differ"
+    e.close()
