@@ -264,6 +264,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_unit_mass") return grav_unit_mass_;
   if (name == "grav_expand") return grav_expand_;
   if (name == "grav_sums") return grav_sums_;
+  if (name == "grav_parts") return grav_parts_;
   if (name == "adj_free_ranges") return adj_free_ranges_;
   if (name == "adj_implicit_h") return adj_implicit_h_;
   if (name == "grav_far") return grav_far_;
@@ -301,6 +302,9 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     grav_unit_mass_ = (int)v;
   } else if (name == "grav_expand") {
     grav_expand_ = (int)v;
+  } else if (name == "grav_parts") {
+    if (!(v >= 0 && v <= 8) || v != (int)v) throw std::invalid_argument("grav_parts must be 0 (automatic) or 1..8");
+    grav_parts_ = (int)v;
   } else if (name == "grav_sums") {
     if (v != -1 && v != 0 && v != 1) throw std::invalid_argument("grav_sums must be -1, 0 or 1");
     grav_sums_ = (int)v;
@@ -680,7 +684,7 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
                            p_.grav_eps,
                            grav_split_only_ | (grav_unit_mass_ ? 0 : 2) | (grav_expand_ ? 0 : 4) |
                                (grav_far_ ? 16 : 0) | (grav_far_ && grav_far_half_ ? 64 : 0),
-                           gs);
+                           gs, grav_parts_);
       return;
     }
   }

@@ -160,7 +160,7 @@ __device__ __forceinline__ void pair_plummer(A* a, T rx, T ry, T rz, T mj, T eps
 constexpr int kGravBlock = 256;
 constexpr int kTile = 64;        // j per blocked-compensation partial (mixed form)
 constexpr int kJTile = 512;      // j records staged in LDS per block iteration (reference-layout kernels)
-constexpr int kGravParts = 8;    // j-parts of the mixed Plummer kernel, one per XCD
+constexpr int kGravParts = 8;    // j-parts of the mixed Plummer kernel, one per XCD (at most)
 // j records per LDS tile of the mixed Plummer kernel: one 64-record sub-tile
 // per wavefront.  Its 12 KB of LDS lets 8 blocks share a CU (the kernel's 63
 // VGPRs allow 8 waves per SIMD), so the 4096 x 8 blocks at N = 2^20 run in
@@ -625,13 +625,14 @@ __device__ __forceinline__ void far_field(const float* __restrict__ M, float yx,
 // partial acceleration, 1 subtraction for the potential.  With every mass
 // equal, tile-form sub-tiles skip r altogether (the expanded body below).
 //
-// Grid: kGravParts j-parts per i-block, block b = i_block * kGravParts + part.
-// Workgroups are dispatched round-robin over the 8 XCDs, so part p runs on
-// XCD p and each XCD streams only its 1/8 of the j-set (4 MiB of hi+lo
-// records at N = 2^20), which its L2 holds; without the split every XCD
-// streams the whole j-set and re-fetches it from HBM.  Each block writes its
-// compensated (hi, lo) partial sums; k_gravity_combine adds the parts in
-// order.
+// Grid: `parts` j-parts per i-block, block b = i_block * parts + part.
+// Workgroups are dispatched round-robin over the 8 XCDs, so with 8 parts
+// part p runs on XCD p and each XCD streams only its 1/8 of the j-set
+// (4 MiB of hi+lo records at N = 2^20), which its L2 holds; without the
+// split every XCD streams the whole j-set and re-fetches it from HBM.  Each
+// block writes its compensated (hi, lo) partial sums; k_gravity_combine adds
+// the parts in order.  (gravity_mixed_parts picks fewer parts where 8 would
+// fill most of the GPU's block slots in one round.)
 // The far-field instantiation holds the 48 moment words of an expansion in
 // VGPRs next to the pairwise state: left alone the compiler takes 127
 // registers (4 wavefronts per SIMD); capped at 5 wavefronts it fits 96
@@ -651,7 +652,7 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
                                                                        FarRatios far_q2,
                                                                        const float* __restrict__ cells,
                                                                        const float* __restrict__ halves,
-                                                                       int n_real) {
+                                                                       int n_real, int parts) {
   // every mass equal (mass_range = {min, max} over the real particles): the
   // pair body drops its mass factor and k_gravity_combine scales by m once;
   // padding records sit at 1e30, where r^2 overflows and rsq returns 0, so
@@ -670,8 +671,8 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
   __shared__ float s_r2[kSub];
   __shared__ __align__(16) float s_mom[kSub][FAR ? kMomWords : 1];
   static_assert(kTile == 64 && JT % kGravBlock == 0, "each wavefront stages whole sub-tiles");
-  const int part = blockIdx.x % kGravParts;
-  const int i = (blockIdx.x / kGravParts) * kGravBlock + threadIdx.x;
+  const int part = blockIdx.x % parts;
+  const int i = (blockIdx.x / parts) * kGravBlock + threadIdx.x;
   const int ii = i < n_i ? i : n_i - 1;
   const int lane = threadIdx.x & 63;
   const F4 pi = hi[ii], li = lo[ii];
@@ -932,17 +933,17 @@ void k_gravity_mixed_plummer(const F4* __restrict__ hi,
   }
 }
 
-// sum of the kGravParts compensated partials of each i, in part order,
-// times the common mass when every mass is equal
+// sum of the `parts` compensated partials of each i, in part order, times
+// the common mass when every mass is equal
 __global__ __launch_bounds__(kBlock) void k_gravity_combine(const F4* __restrict__ part_hi,
                                                             const F4* __restrict__ part_lo, F4* __restrict__ out,
                                                             int n_i, const float* __restrict__ mass_range,
-                                                            int flags) {
+                                                            int flags, int parts) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n_i) return;
   const float m = (flags & 2) == 0 && mass_range[0] == mass_range[1] ? mass_range[0] : 1.f;
   DR a[4] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
-  for (int p = 0; p < kGravParts; ++p) {
+  for (int p = 0; p < parts; ++p) {
     const F4 h = part_hi[(size_t)p * n_i + i], l = part_lo[(size_t)p * n_i + i];
     dr_add(a[0], h.x);
     dr_add(a[0], l.x);
@@ -1139,9 +1140,29 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
   CS_CHECK_LAUNCH();
 }
 
+// j-parts of the pairwise mixed kernel: 8, one per XCD, unless the 8-part
+// grid would take more than 3/8 of the GPU's block slots (8 blocks of
+// 63-register wavefronts per CU) in a single round.  Such a grid holds the
+// CUs until gravity ends and the latency-bound SPH passes queued beside it
+// find no slot; with fewer parts gravity keeps about 3 of each SIMD's 8
+// wavefront slots, which still fill its VALU, and the passes run in the rest.
+// Measured at 65 536 particles (256 i-blocks), mixed step / gravity alone:
+// 8 parts 3.23 / 2.03 ms, 4 parts 2.62 / 2.07, 3 parts 2.38 / 2.22, 2 parts
+// 2.92 / 2.91.  Grids of several rounds keep 8 (262 144 particles: 8 parts
+// 22.6 ms/step, 2 parts 25.0, 1 part 28.0).  The far-field kernel keeps 8.
+int gravity_mixed_parts(int n_i, int far) {
+  if (far) return kGravParts;
+  int dev = 0, cus = 0;
+  CS_HIP(hipGetDevice(&dev));
+  CS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const long slots = 8l * cus, blocks = (long)ceil_div(n_i, kGravBlock);
+  if (blocks * kGravParts <= slots * 3 / 8 || blocks * kGravParts > slots) return kGravParts;
+  return (int)std::max(1l, slots * 3 / 8 / blocks);
+}
+
 void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, float* mass_range,
                           float* moments, float far_ratio, float far_grade, int n_i, int n_j, int n_real,
-                          int soft, float eps, int split_only, hipStream_t s) {
+                          int soft, float eps, int split_only, hipStream_t s, int parts) {
   const dim3 grid(ceil_div(n_i, kGravBlock)), block(kGravBlock);
   if (soft == 0) {
     hipLaunchKernelGGL(k_gravity_mixed_spline, grid, block, 0, s, hi, lo, gvel, out, n_i, n_j);
@@ -1152,7 +1173,10 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
     CS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_mass_range_final, dim3(1), dim3(kBlock), 0, s, mass_range + 2, nb, mass_range);
     CS_CHECK_LAUNCH();
-    const int chunk = ceil_div(ceil_div(n_j, kGravParts), kTile) * kTile;
+    const bool far = (split_only & 16) && moments;
+    if (parts <= 0 || far) parts = gravity_mixed_parts(n_i, far);
+    if (parts > kGravParts) throw std::invalid_argument("gravity: at most 8 j-parts");
+    const int chunk = ceil_div(ceil_div(n_j, parts), kTile) * kTile;
     F4* part_hi = part;
     F4* part_lo = part + (size_t)kGravParts * n_i;
     float* cells = nullptr;
@@ -1191,16 +1215,18 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
       const float q = far_ratio * powf(c, far_grade);
       fq.q2[L] = q * q;
     }
-    const dim3 ggrid(ceil_div(n_i, kGravBlock) * kGravParts);
+    const dim3 ggrid(ceil_div(n_i, kGravBlock) * parts);
     if ((split_only & 16) && moments)
       hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, true>), ggrid, block, 0, s, hi, lo, part_hi, part_lo,
-                         n_i, n_j, chunk, eps * eps, split_only, mass_range, moments, fq, cells, halves, n_real);
+                         n_i, n_j, chunk, eps * eps, split_only, mass_range, moments, fq, cells, halves, n_real,
+                         parts);
     else
       hipLaunchKernelGGL((k_gravity_mixed_plummer<kJTileMixed, false>), ggrid, block, 0, s, hi, lo, part_hi, part_lo,
-                         n_i, n_j, chunk, eps * eps, split_only, mass_range, nullptr, fq, nullptr, nullptr, n_real);
+                         n_i, n_j, chunk, eps * eps, split_only, mass_range, nullptr, fq, nullptr, nullptr, n_real,
+                         parts);
     CS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gravity_combine, dim3(ceil_div(n_i, kBlock)), dim3(kBlock), 0, s, part_hi, part_lo, out,
-                       n_i, mass_range, split_only);
+                       n_i, mass_range, split_only, parts);
   }
   CS_CHECK_LAUNCH();
 }

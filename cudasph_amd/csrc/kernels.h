@@ -71,9 +71,11 @@ void launch_gravity64_plummer(const D4* gpos, D4* out, D4* part, int n_i, int n_
 // field, split_only bit 4, taken where every lane of a wavefront is
 // far_ratio radii away, the ratio at a level of C records scaled by
 // (C / 16384)^far_grade; may be null otherwise)
+// parts: the number of j-parts (1..8), 0 for gravity_mixed_parts(n_i, far)
+int gravity_mixed_parts(int n_i, int far);
 void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, float* mass_range,
                           float* moments, float far_ratio, float far_grade, int n_i, int n_j, int n_real,
-                          int soft, float eps, int split_only, hipStream_t s);
+                          int soft, float eps, int split_only, hipStream_t s, int parts = 0);
 
 // ---- sort.hip ----
 // exclusive scan of in[0..n) into out[0..n]; out[n] = total.  mode 1 scans

@@ -212,7 +212,11 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
  * Mixed-form gravity measurement switches, all results within the form's
  * tolerance: grav_split_only (1: split hi/lo differences for every pair),
  * grav_unit_mass (0: the general-mass body even when every mass is equal),
- * grav_expand (0: the equal-mass tile body on r instead of the expanded one).
+ * grav_expand (0: the equal-mass tile body on r instead of the expanded one),
+ * grav_parts (the pairwise kernel's j-parts, 1..8; 0, the default: 8, or
+ * fewer where 8 would take most of the GPU's block slots in one round -- 3
+ * at 65 536 particles, leaving the SPH passes beside it room on every CU;
+ * the sums' grouping follows it).
  * Plain-form Plummer gravity, the same bits either way: grav_sums (1: each
  * particle's four sums in four wavefronts, 0: in one lane, -1, the default:
  * four wavefronts up to 49 152 particles).

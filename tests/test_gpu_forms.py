@@ -471,3 +471,26 @@ def test_plain_gravity_sums_apart_change_nothing(cs, n, exact):
         out.append(e.get("grav")[:n].copy())
         e.close()
     assert_bits(out[1], out[0], "grav")
+
+
+def test_mixed_gravity_parts_agree(cs):
+    """The pairwise mixed kernel with its j-range in 8, 3 (the default at
+    65 536 particles on a 256-CU GPU) and 1 parts: only the grouping of the compensated sums
+    changes, so the accelerations agree to a few FP32 roundings."""
+    n = 65536
+    e = cs.make_engine("mixed", 4, n, 5, ph_sort=True)
+    out = {}
+    for parts in (8, 3, 1, 0):
+        e.set_scalar("grav_parts", parts)
+        e.run_pass(cs.GRAVITY)
+        out[parts] = e.get("grav")[:n].astype(np.float64)
+    e.close()
+    assert_bits(out[0].astype(np.float32), out[3].astype(np.float32), "default = 3 parts")
+    b = out[8][:, :3]
+    rms = np.sqrt((b ** 2).sum(1).mean())
+    for parts in (3, 1):
+        a = out[parts][:, :3]
+        err = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-3 * rms)
+        assert err.max() < 1e-6, (parts, err.max())
+        pe = np.abs(out[parts][:, 3] - out[8][:, 3]) / np.abs(out[8][:, 3])
+        assert pe.max() < 1e-6, (parts, pe.max())
