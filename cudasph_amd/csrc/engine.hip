@@ -98,7 +98,10 @@ Engine<T>::Engine(int form, const sph_params& p, int device) : form_(form), p_(p
   tree_.alloc(nd_ + 2);
   ties_.alloc(1);
   boxes_.alloc(2 * ((size_t)nd_ + 2));
-  if (!kF32 && gravity64_parts(nd_) > 1) gpart64_.alloc((size_t)gravity64_parts(nd_) * nd_);
+  if (!kF32) {   // the larger of the forward and reverse passes' splits
+    const int parts = std::max(gravity64_parts(nd_), gravity64_forward_parts(nd_));
+    if (parts > 1) gpart64_.alloc((size_t)parts * nd_);
+  }
   hash_.alloc(nd_);
   perm_.alloc(nd_);
   n_ngb_.alloc(nd_);
@@ -690,7 +693,9 @@ void Engine<T>::gravity_kernels(hipStream_t gs) {
   }
   if constexpr (!kF32) {
     if (p_.grav_mode == 1) {
-      launch_gravity64_plummer(gpos_, grav_, gpart64_.p, nd_, nd_, (double)p_.grav_eps, gs);
+      const int parts = grav_parts_ ? grav_parts_ : gravity64_forward_parts(nd_);
+      if (parts > 1 && gpart64_.n < (size_t)parts * nd_) gpart64_.alloc((size_t)parts * nd_);
+      launch_gravity64_plummer(gpos_, grav_, gpart64_.p, nd_, nd_, (double)p_.grav_eps, gs, parts);
       return;
     }
   }

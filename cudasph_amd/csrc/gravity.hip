@@ -1098,8 +1098,25 @@ int gravity64_parts(int n_i) {
   return std::max(1, std::min(8, ceil_div(2048, blocks)));
 }
 
-void launch_gravity64_plummer(const D4* gpos, D4* out, D4* part, int n_i, int n_j, double eps, hipStream_t s) {
-  const int parts = gravity64_parts(n_i);
+// The forward pass runs beside the FP64 SPH passes.  With no more i-blocks
+// than CUs it takes parts for two blocks per CU, leaving the passes the rest
+// of each CU; otherwise the reverse pass's split.  Measured, FP64 step /
+// gravity alone: 65 536 particles, 8 parts 5.51 / 3.54 ms, 3 parts 5.04 /
+// 3.67, 2 parts 4.71 / 3.81; 262 144 particles, 2 parts 50.6 / 50.9 ms,
+// 1 part 70.2 / 54.1 (there the passes beside a half-empty grid slow it).
+int gravity64_forward_parts(int n_i) {
+  int dev = 0, cus = 0;
+  CS_HIP(hipGetDevice(&dev));
+  CS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int blocks = ceil_div(n_i, kGravBlock);
+  if (blocks > cus) return gravity64_parts(n_i);
+  return std::max(1, std::min(8, ceil_div(2 * cus, blocks)));
+}
+
+void launch_gravity64_plummer(const D4* gpos, D4* out, D4* part, int n_i, int n_j, double eps, hipStream_t s,
+                              int parts) {
+  if (parts <= 0) parts = gravity64_forward_parts(n_i);
+  if (parts > 8) throw std::invalid_argument("gravity: at most 8 j-parts");
   const int chunk = ceil_div(ceil_div(n_j, parts), kG64Tile) * kG64Tile;
   hipLaunchKernelGGL(k_gravity64_plummer, dim3(ceil_div(n_i, kGravBlock) * parts), dim3(kGravBlock), 0, s, gpos,
                      parts > 1 ? part : out, n_i, n_j, parts, chunk, eps * eps);

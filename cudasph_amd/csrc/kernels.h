@@ -58,10 +58,14 @@ template <class T>
 void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, int n_j, int soft, T eps,
                     int exact, hipStream_t s, int sums = -1);
 // FP64 form, Plummer softening (fused pair body; the j-range split into
-// gravity64_parts(n_i) parts when the i-blocks alone would not fill the GPU,
-// `part` then holding parts * n_i records)
+// `parts` parts, 0 for gravity64_forward_parts(n_i), `part` then holding
+// parts * n_i records when parts > 1).  gravity64_parts(n_i) is the
+// reverse pass's split: enough parts to fill the GPU when the i-blocks alone
+// would not.
 int gravity64_parts(int n_i);
-void launch_gravity64_plummer(const D4* gpos, D4* out, D4* part, int n_i, int n_j, double eps, hipStream_t s);
+int gravity64_forward_parts(int n_i);
+void launch_gravity64_plummer(const D4* gpos, D4* out, D4* part, int n_i, int n_j, double eps, hipStream_t s,
+                              int parts = 0);
 // part: scratch of 2 * 8 * n_i records (per-j-part partial sums, Plummer only);
 // mass_range: scratch of 2 + 2 * ceil(n_real / 256) floats ({min, max} of the
 // real particles' masses, then per-block partials)

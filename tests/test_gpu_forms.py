@@ -494,3 +494,21 @@ def test_mixed_gravity_parts_agree(cs):
         assert err.max() < 1e-6, (parts, err.max())
         pe = np.abs(out[parts][:, 3] - out[8][:, 3]) / np.abs(out[8][:, 3])
         assert pe.max() < 1e-6, (parts, pe.max())
+
+
+def test_fp64_gravity_parts_agree(cs):
+    """The FP64 forward gravity with its j-range in 8, 2 (the default at
+    65 536 particles on a 256-CU GPU) and 1 parts agrees to FP64 rounding."""
+    n = 65536
+    e = cs.make_engine("fp64", 4, n, 5, ph_sort=True)
+    out = {}
+    for parts in (8, 2, 1, 0):
+        e.set_scalar("grav_parts", parts)
+        e.run_pass(cs.GRAVITY)
+        out[parts] = e.get("grav")[:n].copy()
+    e.close()
+    assert_bits(out[0], out[2], "default = 2 parts")
+    b = out[8]
+    scale = np.abs(b).max(0)
+    for parts in (2, 1):
+        assert (np.abs(out[parts] - b) <= 1e-12 * scale).all(), parts
