@@ -268,6 +268,7 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_expand") return grav_expand_;
   if (name == "grav_sums") return grav_sums_;
   if (name == "grav_parts") return grav_parts_;
+  if (name == "solve_cache") return solve_cache_;
   if (name == "adj_free_ranges") return adj_free_ranges_;
   if (name == "adj_implicit_h") return adj_implicit_h_;
   if (name == "grav_far") return grav_far_;
@@ -305,6 +306,9 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     grav_unit_mass_ = (int)v;
   } else if (name == "grav_expand") {
     grav_expand_ = (int)v;
+  } else if (name == "solve_cache") {
+    if (v != -1 && v != 0 && v != 1) throw std::invalid_argument("solve_cache must be -1, 0 or 1");
+    solve_cache_ = (int)v;
   } else if (name == "grav_parts") {
     if (!(v >= 0 && v <= 8) || v != (int)v) throw std::invalid_argument("grav_parts must be 0 (automatic) or 1..8");
     grav_parts_ = (int)v;
@@ -484,7 +488,12 @@ void Engine<T>::solve() {
   ensure_boxes();
   {
     TimerScope ts(timers_, "solve_range", s_);
-    launch_solve_range<T>(pos_, n_ngb_, hash_, tree_, nd_, walk_params(), s_);
+    WalkParams wp = walk_params();
+    if (solve_cache_ > 0 || (solve_cache_ < 0 && solve_cache_pays(nd_))) {
+      if (solve_rec_.n < (size_t)kSolveCacheK * nd_) solve_rec_.alloc((size_t)kSolveCacheK * nd_);
+      wp.solve_cache = solve_rec_.p;
+    }
+    launch_solve_range<T>(pos_, n_ngb_, hash_, tree_, nd_, wp, s_);
   }
   if (adj_mode_ == kAdjRecord) {
     launch_get_range<T>(pos_, adj_range_.p + rec, nd_, s_);

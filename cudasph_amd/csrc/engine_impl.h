@@ -211,6 +211,8 @@ class Engine final : public EngineBase {
   int grav_unit_mass_ = 1;    // mixed gravity: unit-mass pair body when every mass is equal
   int grav_expand_ = 1;       // mixed gravity: expanded r^2 in the equal-mass tile body
   int grav_parts_ = 0;        // mixed pairwise gravity: j-parts, 0 automatic
+  int solve_cache_ = -1;      // h-solve candidate cache: 1 on, 0 off, -1 where it pays
+  DBuf<V4<T>> solve_rec_;     // its records, kSolveCacheK per particle
   int grav_sums_ = -1;        // plain Plummer gravity: four sums in four wavefronts (1), one lane (0), auto (-1)
   int grav_far_ = 0;          // mixed gravity: order-4 multipole far field for distant sub-tiles
   double grav_far_ratio_ = 8;  // ... taken beyond this many radii of a 16384-record cell

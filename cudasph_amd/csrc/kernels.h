@@ -16,7 +16,11 @@ struct WalkParams {
   // also skips a child whose subtree box lies beyond the range.  The same
   // neighbours in the same order; fewer nodes visited (the walk length)
   const F4* boxes = nullptr;
+  // h-solve candidate cache, or null: kSolveCacheK records of V4<T> per
+  // lane, record k of lane t at k * n_dev + t (launch_solve_range)
+  void* solve_cache = nullptr;
 };
+constexpr int kSolveCacheK = 128;
 
 struct EosConst;  // sph_math.h
 
@@ -24,6 +28,7 @@ struct EosConst;  // sph_math.h
 template <class T>
 void launch_solve_range(V4<T>* pos, int* n_ngb, const KV* hash, const V4<T>* tree, int n_dev,
                         const WalkParams& wp, hipStream_t s);
+bool solve_cache_pays(int n_dev);
 template <class T>
 void launch_find_ngb(const V4<T>* pos, const V4<T>* tree, KV* hash, const int* offset, const int* cap_count,
                      int* count_out, int* list, int* flags, int n_dev, const WalkParams& wp, int clamp,

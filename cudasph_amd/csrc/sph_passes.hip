@@ -37,11 +37,19 @@ namespace cudasph {
 // of dependent fetches does not grow.  The test keeps a margin of 2^-16 of
 // r^2 over the rounding of the box distance and of d^2.
 template <class T>
-__device__ __forceinline__ bool box_within(const F4 lo, const F4 hi, T qx, T qy, T qz, T r2m) {
+__device__ __forceinline__ T box_dist2(const F4 lo, const F4 hi, T qx, T qy, T qz) {
   const T dx = fmax(fmax(T(lo.x) - qx, qx - T(hi.x)), T(0));
   const T dy = fmax(fmax(T(lo.y) - qy, qy - T(hi.y)), T(0));
   const T dz = fmax(fmax(T(lo.z) - qz, qz - T(hi.z)), T(0));
-  return dx * dx + dy * dy + dz * dz <= r2m;
+  return dx * dx + dy * dy + dz * dz;
+}
+template <class T>
+__device__ __forceinline__ bool box_within(const F4 lo, const F4 hi, T qx, T qy, T qz, T r2m) {
+  return box_dist2<T>(lo, hi, qx, qy, qz) <= r2m;
+}
+template <class T>
+__device__ __forceinline__ T range2_margin(T r2) {   // the box test's r^2 with its margin
+  return r2 * T(1.0 + 1.0 / 65536.0);
 }
 
 template <class T, class Visit>
@@ -56,7 +64,7 @@ __device__ __forceinline__ int kd_walk(const V4<T> q, const V4<T>* __restrict__ 
   // becomes a select of addresses, which keeps the caller's copy in scratch
   const T qx = q.x, qy = q.y, qz = q.z, qw = q.w;
   const T r2 = qw * qw;
-  const T r2m = r2 * T(1.0 + 1.0 / 65536.0);
+  const T r2m = range2_margin(r2);
   while (top >= 0) {
     ++n_walk;
     int id = stk[top * stride];
@@ -94,23 +102,162 @@ struct NgbEval {
   T f, step, count;
 };
 
-// smoothed count minus target, Newton step and integer count at range q.w
-// (dev_solve_range.cu:17-89)
+// the sums of one evaluation, neighbour by neighbour in walk order
 template <class T>
-__device__ __forceinline__ NgbEval<T> eval_ngb(const V4<T>& q, const V4<T>* __restrict__ tree,
-                                               const WalkParams& wp, int* stk) {
+struct NgbSum {
   T f = T(0), df = T(0), cnt = T(0);
-  kd_walk<T>(q, tree, wp.n_nodes, wp.fix_phantom, stk, [&](T ds2, int) {
-    const T u = T(2) * dsqrt(ds2) / q.w;
+  __device__ __forceinline__ void add(T ds2, T range) {
+    const T u = T(2) * dsqrt(ds2) / range;
     f += kern_ig<T>(u);
     df -= kern_digdh<T>(u);
     cnt += T(1);
-  }, nullptr, wp.boxes);
-  f -= T(wp.desired_ngb);
-  df = T((double)df * (1.0 / (double)q.w));
-  if (df != T(0)) df = f / df;
-  else df = f < T(0) ? -q.w : q.w;
-  return {f, df, cnt};
+  }
+  __device__ __forceinline__ NgbEval<T> finish(T range, float desired_ngb) {
+    f -= T(desired_ngb);
+    df = T((double)df * (1.0 / (double)range));
+    if (df != T(0)) df = f / df;
+    else df = f < T(0) ? -range : range;
+    return {f, df, cnt};
+  }
+};
+
+// ---------------------------------------------------------------------------
+// h-solve candidate cache (few particles, where a wavefront's walks are
+// latency-bound and its slowest lane's several evaluations set its time).
+// The first evaluation walks at 9/8 of the range and keeps every particle it
+// accepts there, in walk order, with what decides whether a walk at a
+// smaller range r would reach it: the walk reaches a node iff every push on
+// its path passes at r.  A push's split-plane test, fl(qc - r) < nc (or
+// fl(qc + r) > nc), holds for r above the margin qc - nc (nc - qc) up to
+// rounding, so each record carries the path's largest margin m and a bound
+// d >= 8 * eps * max(|qc|, |nc|, range) on the rounding: r >= m + 2d passes
+// every split test, r <= m' - 2d (m' the largest lower bound) fails one.
+// The box tests compare a path's largest box distance^2 b, which does not
+// depend on r, with r^2 (1 + 2^-16): exactly the walk's comparison.  A later
+// evaluation at r <= the cached range filters the records in order: a
+// record inside r whose reach is decided adds its term, and one whose reach
+// falls between the bounds sends that evaluation back to a walk.  So every
+// evaluation sums the terms of the walk at r, in its order: the same bits.
+template <class T>
+struct SolveCache {
+  V4<T>* rec = nullptr;   // this lane's first record, record k at rec[k * stride]; null: no cache
+  int stride = 0;
+  int count = -1;         // -1: not filled; > kSolveCacheK: overflowed (walks only)
+  T radius = T(0);
+};
+
+template <class T>
+__device__ __forceinline__ T round_bound(T a, T b, T c) {
+  constexpr T eps = sizeof(T) == 4 ? T(1.0 / 8388608.0) : T(1.0 / 4503599627370496.0);
+  return T(8) * eps * fmax(fmax(dabs(a), dabs(b)), c);
+}
+
+// kd_walk with the path bounds: the LDS stack holds, per entry, the node and
+// the largest upper (hi) and lower (lo) margin bound and box distance^2 (b)
+// of the pushes on its path.  visit(ds2, hi, lo, b) for each accepted node.
+template <class T, class Visit>
+__device__ __forceinline__ void kd_walk_paths(const V4<T> q, const V4<T>* __restrict__ tree, int n_nodes,
+                                              int fix_phantom, int* __restrict__ stk, T* __restrict__ s_hi,
+                                              T* __restrict__ s_lo, T* __restrict__ s_b, Visit&& visit,
+                                              const F4* __restrict__ boxes) {
+  const int stride = (int)blockDim.x;
+  const T qx = q.x, qy = q.y, qz = q.z, qw = q.w;
+  const T r2 = qw * qw;
+  const T r2m = range2_margin(r2);
+  const T lowest = -qw;   // the root's path: no push, reached at any r > 0
+  int top = 0;
+  stk[0] = 1;
+  s_hi[0] = lowest;
+  s_lo[0] = lowest;
+  s_b[0] = T(0);
+  while (top >= 0) {
+    int id = stk[top * stride];
+    const T hi = s_hi[top * stride], lo = s_lo[top * stride], pb = s_b[top * stride];
+    const V4<T> node = tree[id];
+    F4 b0, b1, b2, b3;
+    if (boxes) {
+      const int cb = 2 * id <= n_nodes ? 4 * id : 4;
+      b0 = boxes[cb];
+      b1 = boxes[cb + 1];
+      b2 = boxes[cb + 2];
+      b3 = boxes[cb + 3];
+    }
+    const int bits = node_bits(node.w);
+    const int split = (bits >> 24) & 3;
+    id <<= 1;
+    if (id <= n_nodes && split < 3) {
+      const T qc = (split & 2) ? qz : ((split & 1) ? qy : qx);
+      const T nc = (split & 2) ? node.z : ((split & 1) ? node.y : node.x);
+      const T d = round_bound<T>(qc, nc, qw);
+      const T bl = boxes ? box_dist2<T>(b0, b1, qx, qy, qz) : T(0);
+      const T br = boxes ? box_dist2<T>(b2, b3, qx, qy, qz) : T(0);
+      const bool push_l = qc - qw < nc && (!boxes || bl <= r2m);
+      const bool push_r = qc + qw > nc && (!fix_phantom || id < n_nodes) && (!boxes || br <= r2m);
+      if (push_l) {
+        stk[top * stride] = id;
+        s_hi[top * stride] = fmax(hi, (qc - nc) + T(2) * d);
+        s_lo[top * stride] = fmax(lo, (qc - nc) - T(2) * d);
+        s_b[top * stride] = fmax(pb, bl);
+        ++top;
+      }
+      if (push_r) {
+        stk[top * stride] = id + 1;
+        s_hi[top * stride] = fmax(hi, (nc - qc) + T(2) * d);
+        s_lo[top * stride] = fmax(lo, (nc - qc) - T(2) * d);
+        s_b[top * stride] = fmax(pb, br);
+        ++top;
+      }
+    }
+    --top;
+    const T dx = qx - node.x, dy = qy - node.y, dz = qz - node.z;
+    const T ds2 = dx * dx + dy * dy + dz * dz;
+    if (ds2 < r2) visit(ds2, hi, lo, pb);
+  }
+}
+
+// the cached records filtered at range r: false when some record's reach is
+// not decided (the caller walks instead)
+template <class T>
+__device__ __forceinline__ bool filter_cache(const SolveCache<T>& c, T r, NgbSum<T>& s) {
+  const T r2 = r * r;
+  const T r2m = range2_margin(r2);
+  for (int k = 0; k < c.count; ++k) {
+    const V4<T> e = c.rec[(size_t)k * c.stride];   // {ds2, hi, lo, b}
+    if (!(e.x < r2)) continue;
+    const bool box_ok = e.w <= r2m;
+    if (box_ok && r >= e.y) s.add(e.x, r);
+    else if (box_ok && r > e.z) return false;
+  }
+  return true;
+}
+
+// smoothed count minus target, Newton step and integer count at range q.w
+// (dev_solve_range.cu:17-89); with a cache, from its records where it can
+template <class T, bool CACHE>
+__device__ __forceinline__ NgbEval<T> eval_ngb(const V4<T>& q, const V4<T>* __restrict__ tree,
+                                               const WalkParams& wp, int* stk, SolveCache<T>& sc) {
+  if (CACHE && sc.count < 0) {   // first evaluation: fill the cache from a walk at 9/8 of the range
+    const int stride = (int)blockDim.x, depth = walk_stack_depth(wp.n_nodes);
+    T* s_hi = reinterpret_cast<T*>(stk - threadIdx.x + depth * stride) + threadIdx.x;
+    T* s_lo = s_hi + depth * stride;
+    T* s_b = s_lo + depth * stride;
+    V4<T> qc = q;
+    qc.w = q.w * T(1.125);
+    int n = 0;
+    kd_walk_paths<T>(qc, tree, wp.n_nodes, wp.fix_phantom, stk, s_hi, s_lo, s_b, [&](T ds2, T hi, T lo, T b) {
+      if (n < kSolveCacheK) sc.rec[(size_t)n * sc.stride] = {ds2, hi, lo, b};
+      ++n;
+    }, wp.boxes);
+    sc.count = n;
+    sc.radius = qc.w;
+  }
+  if (CACHE && sc.count <= kSolveCacheK && q.w <= sc.radius) {
+    NgbSum<T> s;
+    if (filter_cache<T>(sc, q.w, s)) return s.finish(q.w, wp.desired_ngb);
+  }
+  NgbSum<T> s;
+  kd_walk<T>(q, tree, wp.n_nodes, wp.fix_phantom, stk, [&](T ds2, int) { s.add(ds2, q.w); }, nullptr, wp.boxes);
+  return s.finish(q.w, wp.desired_ngb);
 }
 
 template <class T>
@@ -120,11 +267,11 @@ __device__ __forceinline__ T clamp_step(T dx, T range) {   // f_scale = 0.1, dev
 }
 
 // bracket [lo, hi] around the target (dev_solve_range.cu:93-135); z = count
-template <class T>
+template <class T, bool CACHE>
 __device__ __forceinline__ V4<T> bracket(V4<T> q, const V4<T>* __restrict__ tree, const WalkParams& wp,
-                                         int* stk) {
+                                         int* stk, SolveCache<T>& sc) {
   const T rel = T(wp.rel_err);
-  NgbEval<T> f = eval_ngb<T>(q, tree, wp, stk);
+  NgbEval<T> f = eval_ngb<T, CACHE>(q, tree, wp, stk, sc);
   T dx = f.step;
   if (dabs(dx) < rel * q.w)
     return {T((double)q.w - 0.5 * (double)dabs(dx)), T((double)q.w + 0.5 * (double)dabs(dx)), f.count, T(0)};
@@ -135,7 +282,7 @@ __device__ __forceinline__ V4<T> bracket(V4<T> q, const V4<T>* __restrict__ tree
     x_prev = q.w;
     f_prev = f.f;
     q.w -= dx;
-    f = eval_ngb<T>(q, tree, wp, stk);
+    f = eval_ngb<T, CACHE>(q, tree, wp, stk, sc);
     dx = f.step;
     if (dabs(dx) < rel * q.w)
       return {T((double)q.w - 0.5 * (double)dabs(dx)), T((double)q.w + 0.5 * (double)dabs(dx)), f.count, T(0)};
@@ -146,11 +293,11 @@ __device__ __forceinline__ V4<T> bracket(V4<T> q, const V4<T>* __restrict__ tree
 }
 
 // safeguarded Newton inside the bracket (dev_solve_range.cu:137-172)
-template <class T>
+template <class T, bool CACHE>
 __device__ __forceinline__ void solve_range(V4<T> q, const V4<T>* __restrict__ tree, const WalkParams& wp,
-                                            int* stk, T& range, T& count) {
+                                            int* stk, T& range, T& count, SolveCache<T>& sc) {
   const T rel = T(wp.rel_err);
-  V4<T> br = bracket<T>(q, tree, wp, stk);
+  V4<T> br = bracket<T, CACHE>(q, tree, wp, stk, sc);
   T lo = br.x, hi = br.y;
   count = br.z;
   q.w = T(0.5 * (double)(lo + hi));
@@ -164,7 +311,7 @@ __device__ __forceinline__ void solve_range(V4<T> q, const V4<T>* __restrict__ t
         dx = T(0.5 * (double)(hi - lo));
         q.w = T(0.5 * (double)(lo + hi));
       }
-      const NgbEval<T> f = eval_ngb<T>(q, tree, wp, stk);
+      const NgbEval<T> f = eval_ngb<T, CACHE>(q, tree, wp, stk, sc);
       count = f.count;
       if (f.f < T(0)) lo = q.w;
       else hi = q.w;
@@ -176,18 +323,25 @@ __device__ __forceinline__ void solve_range(V4<T> q, const V4<T>* __restrict__ t
 }
 
 // dev_solve_range.cu:174-189
-template <class T>
+template <class T, bool CACHE>
 __global__ __launch_bounds__(kBlock) void k_solve_range(V4<T>* __restrict__ pos, int* __restrict__ n_ngb,
                                                          const KV* __restrict__ hash,
                                                          const V4<T>* __restrict__ tree, int n_dev,
                                                          WalkParams wp) {
-  extern __shared__ int s_stack[];   // walk_stack_depth(n_nodes) entries per lane
+  // walk_stack_depth(n_nodes) entries per lane; with the cache, three
+  // columns of T per entry after them (the path bounds)
+  extern __shared__ int s_stack[];
   const int t = xcd_block(blockIdx.x, gridDim.x) * (int)blockDim.x + threadIdx.x;
   if (t >= n_dev) return;
   const int i = (int)hash[t].value;
   const V4<T> q = pos[i];
   T range = T(0), count = T(0);
-  if (q.w > T(0)) solve_range<T>(q, tree, wp, s_stack + threadIdx.x, range, count);
+  SolveCache<T> sc;
+  if (CACHE) {
+    sc.rec = static_cast<V4<T>*>(wp.solve_cache) + t;
+    sc.stride = n_dev;
+  }
+  if (q.w > T(0)) solve_range<T, CACHE>(q, tree, wp, s_stack + threadIdx.x, range, count, sc);
   pos[i].w = range;
   n_ngb[i] = (int)count;
 }
@@ -584,13 +738,24 @@ __global__ __launch_bounds__(kBlock) void k_hydro_mixed(const F4* __restrict__ p
 // over every CU instead of sharing the memory pipelines of a quarter of them
 static int walk_block(int n_dev) { return n_dev < 256 * kBlock ? 64 : kBlock; }
 
+// the candidate cache where the walks run one wavefront per CU
+bool solve_cache_pays(int n_dev) { return walk_block(n_dev) == 64; }
+
 template <class T>
 void launch_solve_range(V4<T>* pos, int* n_ngb, const KV* hash, const V4<T>* tree, int n_dev,
                         const WalkParams& wp, hipStream_t s) {
   const int bs = walk_block(n_dev);
-  const size_t lds = (size_t)walk_stack_depth(wp.n_nodes) * bs * sizeof(int);
-  hipLaunchKernelGGL(k_solve_range<T>, dim3(ceil_div(n_dev, bs)), dim3(bs), lds, s, pos, n_ngb, hash, tree, n_dev,
-                     wp);
+  // the cache's path bounds take three more LDS columns; past 64 KB a block
+  // walks without it
+  const size_t stack = (size_t)walk_stack_depth(wp.n_nodes) * bs;
+  const bool cache = wp.solve_cache && stack * (sizeof(int) + 3 * sizeof(T)) <= 65536;
+  const size_t lds = stack * (sizeof(int) + (cache ? 3 * sizeof(T) : 0));
+  if (cache)
+    hipLaunchKernelGGL((k_solve_range<T, true>), dim3(ceil_div(n_dev, bs)), dim3(bs), lds, s, pos, n_ngb, hash, tree,
+                       n_dev, wp);
+  else
+    hipLaunchKernelGGL((k_solve_range<T, false>), dim3(ceil_div(n_dev, bs)), dim3(bs), lds, s, pos, n_ngb, hash,
+                       tree, n_dev, wp);
   CS_CHECK_LAUNCH();
 }
 

@@ -217,6 +217,8 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
  * fewer where 8 would take most of the GPU's block slots in one round -- 3
  * at 65 536 particles, leaving the SPH passes beside it room on every CU;
  * the sums' grouping follows it; also the FP64 form's forward gravity).
+ * solve_cache (the h-solve's candidate cache: 1 on, 0 off, -1, the default:
+ * on below 65 536 particles; the same bits either way).
  * Plain-form Plummer gravity, the same bits either way: grav_sums (1: each
  * particle's four sums in four wavefronts, 0: in one lane, -1, the default:
  * four wavefronts up to 49 152 particles).

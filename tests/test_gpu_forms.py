@@ -512,3 +512,27 @@ def test_fp64_gravity_parts_agree(cs):
     scale = np.abs(b).max(0)
     for parts in (2, 1):
         assert (np.abs(out[parts] - b) <= 1e-12 * scale).all(), parts
+
+
+@pytest.mark.parametrize("form,cfg,n", [("plain", 1, 16384), ("plain", 3, 5000), ("mixed", 1, 16384),
+                                        ("mixed", 2, 30001), ("fp64", 4, 12000), ("plain", 2, 70000), ("fp64", 2, 40000)])
+def test_solve_cache_changes_nothing(cs, form, cfg, n):
+    """The h-solve with its candidate cache (one walk at 9/8 of the range,
+    later evaluations filtered from it, or walked where a record's reach is
+    not decided) against the walks alone: the same ranges and counts, and
+    three whole steps the same in every bit."""
+    print(f"solve cache {form} cfg={cfg} n={n}", flush=True)
+    out = []
+    for cache in (0, 1):
+        e = cs.make_engine(form, cfg, n, 17, ph_sort=True)
+        e.set_scalar("solve_cache", cache)
+        e.init()
+        r = [(e.get("pos")[:n, 3].copy(), e.get("n_ngb")[:n].copy())]
+        e.step(3)
+        r.append({k: e.get(k)[:n].copy() for k in ("pos", "vel", "hydro", "bdots", "n_ngb")})
+        out.append(r)
+        e.close()
+    assert_bits(out[1][0][0], out[0][0][0], "init ranges")
+    assert np.array_equal(out[1][0][1], out[0][0][1]), "init counts"
+    for k in out[0][1]:
+        assert_bits(out[1][1][k], out[0][1][k], k)
