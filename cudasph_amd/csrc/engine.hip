@@ -268,6 +268,10 @@ double Engine<T>::get_scalar(const std::string& name) {
   if (name == "grav_expand") return grav_expand_;
   if (name == "grav_sums") return grav_sums_;
   if (name == "grav_parts") return grav_parts_;
+  if (name == "grav_parts_auto") {   // the j-parts the automatic choice takes at this size
+    if (!kF32) return gravity64_forward_parts(nd_);
+    return mixed() ? gravity_mixed_parts(nd_, grav_far_) : 1;
+  }
   if (name == "solve_cache") return solve_cache_;
   if (name == "adj_free_ranges") return adj_free_ranges_;
   if (name == "adj_implicit_h") return adj_implicit_h_;

@@ -216,7 +216,8 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
  * grav_parts (the pairwise kernel's j-parts, 1..8; 0, the default: 8, or
  * fewer where 8 would take most of the GPU's block slots in one round -- 3
  * at 65 536 particles, leaving the SPH passes beside it room on every CU;
- * the sums' grouping follows it; also the FP64 form's forward gravity).
+ * the sums' grouping follows it; also the FP64 form's forward gravity;
+ * read-only grav_parts_auto: the count the default takes at this size).
  * solve_cache (the h-solve's candidate cache: 1 on, 0 off, -1, the default:
  * on below 65 536 particles; the same bits either way).
  * Plain-form Plummer gravity, the same bits either way: grav_sums (1: each
