@@ -317,7 +317,7 @@ void Engine<T>::set_scalar(const std::string& name, double v) {
     if (!(v >= 0 && v <= 8) || v != (int)v) throw std::invalid_argument("grav_parts must be 0 (automatic) or 1..8");
     grav_parts_ = (int)v;
   } else if (name == "grav_sums") {
-    if (v != -1 && v != 0 && v != 1) throw std::invalid_argument("grav_sums must be -1, 0 or 1");
+    if (v != -1 && v != 0 && v != 1 && v != 2) throw std::invalid_argument("grav_sums must be -1, 0, 1 or 2");
     grav_sums_ = (int)v;
   } else if (name == "adj_free_ranges") {
     adj_free_ranges_ = (int)v;

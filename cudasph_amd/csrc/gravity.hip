@@ -293,8 +293,7 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_lane(const F4* __restric
 // running one two-sum per pair instead of four (~23 VALU + rsq per pair and
 // lane against ~58 + rsq), on four times as many wavefronts: at N = 16 384
 // the one-lane kernel has a wavefront for 256 of the GPU's 1024 SIMDs, this
-// one fills all of them, 8 records ahead (16 384: 2.19 -> 0.89 ms; 49 152:
-// 6.55 -> 5.16 ms; 65 536: 8.9 ms against the one-lane kernel's 7.4).
+// one fills all of them, 8 records ahead (16 384: 2.19 -> 0.89 ms).
 constexpr int kSumParticles = kGravBlock / 4;   // particles per block of k_gravity_sums
 
 template <int C, bool EXACT>
@@ -322,6 +321,51 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_sums(const F4* __restric
     }
   }
   if (i < n_i) reinterpret_cast<float*>(out + i)[c] = acc.hi;
+}
+
+// Between the two: two wavefronts per 64 particles, one keeping ax and ay,
+// the other az and the potential, each repeating the pair's distance and
+// rsq once for its two two-sums, 4 records ahead.  Measured (ms per launch,
+// one lane / four wavefronts / two wavefronts): 16 384 particles 1.85 /
+// 0.89 / 1.23; 20 480: 2.31 / 1.56 / 1.53; 32 768: 3.71 / 2.54 / 2.46;
+// 49 152: 5.56 / 5.16 / 4.42; 65 536: 7.43 / 8.9 / 5.9; 81 920: 9.25 / 13.4 /
+// 12.7.
+template <int C0, int C1, bool EXACT>
+__device__ __forceinline__ void plummer_two(DR& a0, DR& a1, const F4* __restrict__ s_pos, int cnt, const F4& pi,
+                                            float eps2) {
+  pipelined<4>(
+      cnt,
+      [&](int j) {
+        const F4 pj = s_pos[j];
+        return float2{plummer_term<C0, EXACT>(pj, pi, eps2), plummer_term<C1, EXACT>(pj, pi, eps2)};
+      },
+      [&](const float2& y) {
+        dr_sub(a0, y.x);
+        dr_sub(a1, y.y);
+      });
+}
+
+constexpr int kTwoParticles = kGravBlock / 2;   // particles per block of k_gravity_two
+
+template <bool EXACT>
+__global__ __launch_bounds__(kGravBlock) void k_gravity_two(const F4* __restrict__ gpos, F4* __restrict__ out,
+                                                            int n_i, int n_j, float eps2) {
+  __shared__ F4 s_pos[kJTile];
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int half = w & 1;   // 0: ax, ay; 1: az, potential
+  const int i = blockIdx.x * kTwoParticles + (w >> 1) * 64 + (threadIdx.x & 63);
+  const F4 pi = gpos[i < n_i ? i : n_i - 1];
+  DR a0{0.f, 0.f}, a1{0.f, 0.f};
+  for (int t0 = 0; t0 < n_j; t0 += kJTile) {
+    const int cnt = stage_tile(s_pos, gpos, t0, n_j);
+    if (half == 0) plummer_two<0, 1, EXACT>(a0, a1, s_pos, cnt, pi, eps2);
+    else plummer_two<2, 3, EXACT>(a0, a1, s_pos, cnt, pi, eps2);
+  }
+  if (i < n_i) {
+    float* o = reinterpret_cast<float*>(out + i) + 2 * half;
+    o[0] = a0.hi;
+    o[1] = a1.hi;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1127,8 +1171,9 @@ void launch_gravity64_plummer(const D4* gpos, D4* out, D4* part, int n_i, int n_
   }
 }
 
-bool gravity_sums_apart(int n_i, int sums) {
-  return sums >= 0 ? sums != 0 : n_i <= kSumsApartMaxN;
+int gravity_sums_mode(int n_i, int sums) {
+  if (sums >= 0) return sums;
+  return n_i <= kFourSumsMaxN ? 1 : (n_i <= kTwoSumsMaxN ? 2 : 0);
 }
 
 template <class T>
@@ -1138,7 +1183,12 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
   const T eps2 = eps * eps;
   if constexpr (sizeof(T) == 4) {
     if (soft == 1) {
-      if (gravity_sums_apart(n_i, sums)) {
+      const int mode = gravity_sums_mode(n_i, sums);
+      if (mode == 2) {
+        const dim3 grid2(ceil_div(n_i, kTwoParticles));
+        if (exact) hipLaunchKernelGGL(k_gravity_two<true>, grid2, block, 0, s, gpos, out, n_i, n_j, eps2);
+        else hipLaunchKernelGGL(k_gravity_two<false>, grid2, block, 0, s, gpos, out, n_i, n_j, eps2);
+      } else if (mode == 1) {
         const dim3 grid4(ceil_div(n_i, kSumParticles));
         if (exact) hipLaunchKernelGGL(k_gravity_sums<true>, grid4, block, 0, s, gpos, out, n_i, n_j, eps2);
         else hipLaunchKernelGGL(k_gravity_sums<false>, grid4, block, 0, s, gpos, out, n_i, n_j, eps2);

@@ -55,10 +55,12 @@ void launch_hydro(const V4<T>* pos, const V4<T>* vel, const V4<T>* hydro, const 
 
 // ---- gravity.hip ----
 // sums: for the FP32 Plummer kernel, 1 keeps each particle's four sums in
-// four wavefronts (k_gravity_sums), 0 in one lane, -1 (auto) four
-// wavefronts up to kSumsApartMaxN particles; the same bits either way
-constexpr int kSumsApartMaxN = 49152;
-bool gravity_sums_apart(int n_i, int sums);
+// four wavefronts (k_gravity_sums), 2 in two (k_gravity_two), 0 in one lane
+// (k_gravity_lane), -1 (auto) four up to kFourSumsMaxN particles, two up to
+// kTwoSumsMaxN, one lane above; the same bits either way
+constexpr int kFourSumsMaxN = 16384;
+constexpr int kTwoSumsMaxN = 65536;
+int gravity_sums_mode(int n_i, int sums);
 template <class T>
 void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, int n_j, int soft, T eps,
                     int exact, hipStream_t s, int sums = -1);

@@ -452,16 +452,17 @@ def test_mixed_state_file_keeps_fp64_state(cs, tmp_path):
 
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 65, 4099, 16384, 49152, 49153])
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 127, 129, 4099, 16384, 16385, 65536, 65537])
 @pytest.mark.parametrize("exact", [0, 1])
 def test_plain_gravity_sums_apart_change_nothing(cs, n, exact):
     """The plain form's Plummer gravity with each particle's four sums in
-    four wavefronts gives the bits of the one-lane kernel (hardware and
-    correctly rounded rsqrt), at sizes around the 64-particle blocks and on
-    either side of the automatic switch at 49 152."""
+    four wavefronts and in two gives the bits of the one-lane kernel
+    (hardware and correctly rounded rsqrt), at sizes around the 64- and
+    128-particle blocks and on either side of the automatic switches at
+    16 384 and 65 536."""
     print(f"plain gravity sums n={n} exact={exact}", flush=True)
     out = []
-    for sums in (0, 1):
+    for sums in (0, 1, 2, -1):
         e = cs.make_engine("plain", 1, n, 4, ph_sort=True)
         assert e.params.grav_mode == 1
         e.set_scalar("exact_rsqrt", exact)
@@ -470,7 +471,8 @@ def test_plain_gravity_sums_apart_change_nothing(cs, n, exact):
         e.run_pass(cs.GRAVITY)
         out.append(e.get("grav")[:n].copy())
         e.close()
-    assert_bits(out[1], out[0], "grav")
+    for k, name in ((1, "four wavefronts"), (2, "two wavefronts"), (3, "automatic")):
+        assert_bits(out[k], out[0], f"grav ({name})")
 
 
 @pytest.mark.parametrize("n", [65536, 40001])

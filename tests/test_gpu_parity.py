@@ -181,13 +181,13 @@ def test_plummer_ten_steps_lists_bit_exact(cs, orc):
         assert_bits(e.get(name)[:n], o.get(name)[:n], name)
 
 
-@pytest.mark.parametrize("mode,sums", [(0, -1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("mode,sums", [(0, -1), (1, 0), (1, 1), (1, 2)])
 @pytest.mark.parametrize("cfg,n", [(3, 4099), (2, 1037), (1, 513), (1, 5)])
 def test_plain_gravity_bit_exact_ragged(cs, orc, cfg, n, mode, sums):
     """Gravity alone at sizes that leave a partial LDS tile with an odd
     record count and a partial block of i-particles, spline and Plummer
-    softening (the Plummer kernel with each particle's four sums in one lane
-    and in four wavefronts): bit-identical to the oracle."""
+    softening (the Plummer kernel with each particle's four sums in one lane,
+    in four wavefronts and in two): bit-identical to the oracle."""
     print(f"plain gravity cfg={cfg} n={n} mode={mode} sums={sums}", flush=True)
     e, o, _ = make_pair(cs, orc, cfg, n, grav_mode=mode)
     e.set_scalar("exact_rsqrt", 1)
