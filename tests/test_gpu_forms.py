@@ -525,7 +525,7 @@ def test_fp64_gravity_parts_agree(cs, n):
 
 
 @pytest.mark.parametrize("form,cfg,n", [("plain", 1, 16384), ("plain", 3, 5000), ("mixed", 1, 16384),
-                                        ("mixed", 2, 30001), ("fp64", 4, 12000), ("plain", 2, 70000), ("fp64", 2, 40000)])
+                                        ("mixed", 2, 30001), ("fp64", 4, 12000), ("plain", 2, 60000), ("fp64", 2, 40000)])
 def test_solve_cache_changes_nothing(cs, form, cfg, n):
     """The h-solve with its candidate cache (one walk at 9/8 of the range,
     later evaluations filtered from it, or walked where a record's reach is
