@@ -194,7 +194,7 @@ def test_compute_cuda_gravity_drop_in(cs, orc, mode):
 def test_radix_sort_is_stable(cs):
     L = cs.lib()
     rng = np.random.default_rng(3)
-    for n, bits in ((1, 8), (1000, 8), (70001, 20), (300000, 32)):
+    for n, bits in ((1, 8), (5, 32), (1000, 8), (20000, 32), (32768, 32), (32769, 24), (70001, 20), (300000, 32)):
         keys = rng.integers(0, 1 << min(bits, 16), n, dtype=np.uint64).astype(np.uint32)
         if bits == 32:
             keys = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
