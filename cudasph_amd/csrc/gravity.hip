@@ -329,10 +329,9 @@ __global__ __launch_bounds__(kGravBlock) void k_gravity_sums(const F4* __restric
 // launch, one lane / four wavefronts / two wavefronts): 16 384 particles
 // 1.85 / 0.89 / 1.23; 20 480: 2.31 / 1.56 / 1.53; 32 768: 3.71 / 2.54 / 2.46;
 // 49 152: 5.56 / 5.16 / 4.42; 65 536: 7.43 / 8.9 / 5.9.  In the plain step,
-// beside the SPH passes, fewer wavefronts leave them more room (ms/step):
-// 16 384: 2.03 / 1.41 / 1.44; 24 576: 2.86 / 1.96 / 1.93; 32 768: 4.07 /
-// 4.23 / 3.50; 49 152: 5.65 / 6.44 / 6.04; 65 536: 7.52 / 9.63 / 8.67; so
-// the automatic choice takes two wavefronts only up to 40 960 particles.
+// beside the SPH passes (whose walks take the higher issue priority), ms per
+// step: 16 384: 2.05 / 1.33 / 1.45; 32 768: 4.10 / 3.84 / 3.38; 49 152:
+// 5.72 / - / 5.26; 65 536: 7.62 / - / 6.72; 81 920: 10.47 / - / 13.2.
 template <int C0, int C1, bool EXACT>
 __device__ __forceinline__ void plummer_two(DR& a0, DR& a1, const F4* __restrict__ s_pos, int cnt, const F4& pi,
                                             float eps2) {

@@ -59,7 +59,7 @@ void launch_hydro(const V4<T>* pos, const V4<T>* vel, const V4<T>* hydro, const 
 // (k_gravity_lane), -1 (auto) four up to kFourSumsMaxN particles, two up to
 // kTwoSumsMaxN, one lane above; the same bits either way
 constexpr int kFourSumsMaxN = 16384;
-constexpr int kTwoSumsMaxN = 40960;
+constexpr int kTwoSumsMaxN = 65536;
 int gravity_sums_mode(int n_i, int sums);
 template <class T>
 void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, int n_j, int soft, T eps,

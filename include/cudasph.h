@@ -222,7 +222,7 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
  * on below 65 536 particles; the same bits either way).
  * Plain-form Plummer gravity, the same bits either way: grav_sums (1: each
  * particle's four sums in four wavefronts, 2: in two, 0: in one lane; -1,
- * the default: four wavefronts up to 16 384 particles, two up to 40 960).
+ * the default: four wavefronts up to 16 384 particles, two up to 65 536).
  * Mixed-form gravity options: grav_far (1: an order-4 multipole far field for
  * each cell of 16384, 4096, 1024 or 256 records, or 64-record sub-tile, that a
  * whole wavefront sees from beyond grav_far_ratio of its radii, default 8,

@@ -452,14 +452,14 @@ def test_mixed_state_file_keeps_fp64_state(cs, tmp_path):
 
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 65, 127, 129, 4099, 16384, 16385, 40960, 40961, 65537])
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 127, 129, 4099, 16384, 16385, 65536, 65537])
 @pytest.mark.parametrize("exact", [0, 1])
 def test_plain_gravity_sums_apart_change_nothing(cs, n, exact):
     """The plain form's Plummer gravity with each particle's four sums in
     four wavefronts and in two gives the bits of the one-lane kernel
     (hardware and correctly rounded rsqrt), at sizes around the 64- and
     128-particle blocks and on either side of the automatic switches at
-    16 384 and 40 960."""
+    16 384 and 65 536."""
     print(f"plain gravity sums n={n} exact={exact}", flush=True)
     out = []
     for sums in (0, 1, 2, -1):
