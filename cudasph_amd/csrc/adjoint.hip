@@ -897,6 +897,23 @@ void Engine<T>::adj_forces_backward(bool seed_rho) {
     throw std::runtime_error("adjoint: FP64 form only");
   } else {
     if (pairbuf_.n < (size_t)ngb_tot_ * kPair) pairbuf_.alloc((size_t)ngb_tot_ * kPair + 64);
+    // gravity's part sums depend only on x, m and a_bar, so with several
+    // parts they run on the second stream beside the SPH kernels, and their
+    // combine adds them to x_bar after those kernels, as in stream order
+    const double eps = (double)p_.grav_eps;
+    const int parts = grav_parts_ ? grav_parts_ : gravity64_parts(n_);
+    const int chunk = ceil_div(ceil_div(n_, parts), kAdjTile) * kAdjTile;
+    const bool beside = overlap_ && parts > 1;
+    if (parts > 1 && gpart64_.n < (size_t)parts * nd_) gpart64_.alloc((size_t)parts * nd_);
+    if (beside) {
+      CS_HIP(hipEventRecord(ev_fork_, s_));
+      CS_HIP(hipStreamWaitEvent(gs_, ev_fork_, 0));
+      TimerScope tg(timers_, "adjoint_gravity", gs_);
+      hipLaunchKernelGGL(k_adj_gravity, dim3(ceil_div(n_, kBlock) * parts), dim3(kBlock), 0, gs_, pos_.p, vel_.p,
+                         aa_.p, xs_.p, (D4v*)gpart64_.p, n_, parts, chunk, eps * eps);
+      CS_CHECK_LAUNCH();
+      CS_HIP(hipEventRecord(ev_join_, gs_));
+    }
     TimerScope ts(timers_, "adjoint_sph", s_);
     hipLaunchKernelGGL(k_adj_hydro, grid(n_), dim3(kBlock), 0, s_, pos_.p, vel_.p, hydro_.p, dots_.p, aa_.p,
                        ngb_offset_.p, ngb_count_.p, list_.p, xs_.p, vs_.p, hb1_.p, hb2_.p, pairbuf_.p, n_,
@@ -931,13 +948,14 @@ void Engine<T>::adj_forces_backward(bool seed_rho) {
       CS_CHECK_LAUNCH();
     }
     ts.t.end(s_);
-    ts.t.begin("adjoint_gravity", s_);
-    const double eps = (double)p_.grav_eps;
-    const int parts = gravity64_parts(n_);
-    const int chunk = ceil_div(ceil_div(n_, parts), kAdjTile) * kAdjTile;
-    hipLaunchKernelGGL(k_adj_gravity, dim3(ceil_div(n_, kBlock) * parts), dim3(kBlock), 0, s_, pos_.p, vel_.p, aa_.p,
-                       xs_.p, (D4v*)gpart64_.p, n_, parts, chunk, eps * eps);
-    CS_CHECK_LAUNCH();
+    if (beside) {
+      CS_HIP(hipStreamWaitEvent(s_, ev_join_, 0));
+    } else {
+      ts.t.begin("adjoint_gravity", s_);
+      hipLaunchKernelGGL(k_adj_gravity, dim3(ceil_div(n_, kBlock) * parts), dim3(kBlock), 0, s_, pos_.p, vel_.p,
+                         aa_.p, xs_.p, (D4v*)gpart64_.p, n_, parts, chunk, eps * eps);
+      CS_CHECK_LAUNCH();
+    }
     if (parts > 1) {
       hipLaunchKernelGGL(k_adj_gravity_combine, grid(n_), dim3(kBlock), 0, s_, (const D4v*)gpart64_.p, xs_.p, n_,
                          parts);

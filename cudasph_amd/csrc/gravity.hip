@@ -1210,23 +1210,24 @@ void launch_gravity(const V4<T>* gpos, const V4<T>* gvel, V4<T>* out, int n_i, i
 }
 
 // j-parts of the pairwise mixed kernel: 8, one per XCD, unless the 8-part
-// grid would take more than 3/8 of the GPU's block slots (8 blocks of
+// grid would take more than half of the GPU's block slots (8 blocks of
 // 63-register wavefronts per CU) in a single round.  Such a grid holds the
 // CUs until gravity ends and the latency-bound SPH passes queued beside it
-// find no slot; with fewer parts gravity keeps about 3 of each SIMD's 8
-// wavefront slots, which still fill its VALU, and the passes run in the rest.
-// Measured at 65 536 particles (256 i-blocks), mixed step / gravity alone:
-// 8 parts 3.23 / 2.03 ms, 4 parts 2.62 / 2.07, 3 parts 2.38 / 2.22, 2 parts
-// 2.92 / 2.91.  Grids of several rounds keep 8 (262 144 particles: 8 parts
-// 22.6 ms/step, 2 parts 25.0, 1 part 28.0).  The far-field kernel keeps 8.
+// find no slot; with fewer parts gravity keeps about half of each SIMD's 8
+// wavefront slots, which still fill its VALU, and the passes run in the rest
+// (their walks at the higher issue priority).  Mixed step at 65 536
+// particles (256 i-blocks): 8 parts 3.21 ms, 4 parts 2.19, 3 parts 2.39
+// (gravity alone 2.03 / 2.07 / 2.22 ms).  Grids of several rounds keep 8
+// (262 144 particles: 8 parts 22.6 ms/step, 2 parts 25.0, 1 part 28.0).  The
+// far-field kernel keeps 8.
 int gravity_mixed_parts(int n_i, int far) {
   if (far) return kGravParts;
   int dev = 0, cus = 0;
   CS_HIP(hipGetDevice(&dev));
   CS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const long slots = 8l * cus, blocks = (long)ceil_div(n_i, kGravBlock);
-  if (blocks * kGravParts <= slots * 3 / 8 || blocks * kGravParts > slots) return kGravParts;
-  return (int)std::max(1l, slots * 3 / 8 / blocks);
+  if (blocks * kGravParts <= slots / 2 || blocks * kGravParts > slots) return kGravParts;
+  return (int)std::max(1l, slots / 2 / blocks);
 }
 
 void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F4* part, float* mass_range,

@@ -214,7 +214,7 @@ long   sph_engine_get(void* e, const char* name, void* host, long max_bytes);
  * grav_unit_mass (0: the general-mass body even when every mass is equal),
  * grav_expand (0: the equal-mass tile body on r instead of the expanded one),
  * grav_parts (the pairwise kernel's j-parts, 1..8; 0, the default: 8, or
- * fewer where 8 would take most of the GPU's block slots in one round -- 3
+ * fewer where 8 would take most of the GPU's block slots in one round -- 4
  * at 65 536 particles, leaving the SPH passes beside it room on every CU;
  * the sums' grouping follows it; also the FP64 form's forward gravity;
  * read-only grav_parts_auto: the count the default takes at this size).

@@ -478,7 +478,7 @@ def test_plain_gravity_sums_apart_change_nothing(cs, n, exact):
 @pytest.mark.parametrize("n", [65536, 40001])
 def test_mixed_gravity_parts_agree(cs, n):
     """The pairwise mixed kernel with its j-range in 8, 3 and 1 parts and the
-    automatic count (3 at 65 536 particles, 4 at 40 001 on a 256-CU GPU): only
+    automatic count (4 at 65 536 particles, 6 at 40 001 on a 256-CU GPU): only
     the grouping of the compensated sums changes, so the accelerations agree
     within the form's tolerance (1e-5 of max(|a|, 1e-3 rms); particles whose
     forces nearly cancel differ most) and the potentials to 1e-6, and the
