@@ -94,6 +94,9 @@ void launch_gravity_mixed(const F4* hi, const F4* lo, const F4* gvel, F4* out, F
 // scan_scratch_ints(n) ints.
 long scan_scratch_ints(long n);
 void launch_exclusive_scan(const int* in, int* out, long n, int mode, int* scratch, hipStream_t s);
+// two such scans (mode 0) in one launch sequence: the second's input and
+// output `stride` ints after the first's; scratch 2 scan_scratch_ints(n)
+void launch_exclusive_scan_pair(const int* in, int* out, long n, long stride, int* scratch, hipStream_t s);
 // stable LSD radix sort of KV records by key bits [0, bits); the result is
 // left in `data` (alt is scratch of the same length).  scratch holds
 // radix_scratch_ints(n) ints.
@@ -132,7 +135,7 @@ void launch_ph_keys(const V4<T>* pos, int n, int bits, uint32_t* key_hi, uint32_
 // device: tree[0..n_dev+1] and hash[0..n_dev) as the host builder writes them
 // (heap_order 1: hash in the nodes' heap order, 0: identity).  work holds
 // tree_work_ints(n) ints; kv and kv_alt 3 n records each; sort_scratch
-// 3 radix_scratch_ints(n) and scan_scratch scan_scratch_ints(n) ints.
+// 3 radix_scratch_ints(n) and scan_scratch 2 scan_scratch_ints(n) ints.
 long tree_work_ints(long n);
 // ties (may be null): set to 1 when some node's particle shares its split
 // coordinate with another particle of its subtree -- the one case in which

@@ -43,14 +43,14 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_tmp, int& tota
   return incl - v;
 }
 
-// Batched scans (blockIdx.y = batch b): array b's input, output and tile
-// partials start `stride` ints after array b-1's (one stride: each batch
-// lives at the same offsets inside its own stretch of memory).
+// Batched scans (blockIdx.y = batch b): array b's input and output start
+// `stride` ints after array b-1's, its tile partials `pstride` ints after
+// b-1's.
 __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int* __restrict__ in, long n, int mode,
-                                                             int* __restrict__ partial, long stride) {
+                                                             int* __restrict__ partial, long stride, long pstride) {
   __shared__ int s_tmp[kScanBlock];
   in += blockIdx.y * stride;
-  partial += blockIdx.y * stride;
+  partial += blockIdx.y * pstride;
   const long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
   int sum = 0;
   for (int k = 0; k < kScanItems; ++k) sum += scan_load(in, base + k, n, mode);
@@ -60,9 +60,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int* __restric
 }
 
 // one block scans the tile partials in chunks of 256, carrying the sum
-__global__ __launch_bounds__(kScanBlock) void k_scan_partials(int* __restrict__ partial, int n_part, long stride) {
+__global__ __launch_bounds__(kScanBlock) void k_scan_partials(int* __restrict__ partial, int n_part, long pstride) {
   __shared__ int s_tmp[kScanBlock];
-  partial += blockIdx.y * stride;
+  partial += blockIdx.y * pstride;
   int carry = 0;
   for (int c0 = 0; c0 < n_part; c0 += kScanBlock) {
     const int idx = c0 + threadIdx.x;
@@ -77,10 +77,10 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(int* __restrict__ 
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int* __restrict__ in, long n, int mode,
                                                            const int* __restrict__ partial, int* __restrict__ out,
-                                                           int n_part, long stride) {
+                                                           int n_part, long stride, long pstride) {
   __shared__ int s_tmp[kScanBlock];
   in += blockIdx.y * stride;
-  partial += blockIdx.y * stride;
+  partial += blockIdx.y * pstride;
   out += blockIdx.y * stride;
   const long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
   int v[kScanItems];
@@ -100,26 +100,31 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int* __restrict_
 
 long scan_scratch_ints(long n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
-// `batch` scans at once: in, out and scratch of scan b at b * stride ints
-// past the first's
+// `batch` scans at once: in and out of scan b at b * stride ints past the
+// first's, its scratch at b * pstride
 static void exclusive_scan_batch(const int* in, int* out, long n, int mode, int* scratch, hipStream_t s,
-                                 int batch, long stride) {
+                                 int batch, long stride, long pstride) {
   const int n_part = (int)((n + kScanTile - 1) / kScanTile);
   if (n_part == 0) {
     for (int b = 0; b < batch; ++b) CS_HIP(hipMemsetAsync(out + b * stride, 0, sizeof(int), s));
     return;
   }
-  hipLaunchKernelGGL(k_scan_reduce, dim3(n_part, batch), dim3(kScanBlock), 0, s, in, n, mode, scratch, stride);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(n_part, batch), dim3(kScanBlock), 0, s, in, n, mode, scratch, stride,
+                     pstride);
   CS_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_scan_partials, dim3(1, batch), dim3(kScanBlock), 0, s, scratch, n_part, stride);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1, batch), dim3(kScanBlock), 0, s, scratch, n_part, pstride);
   CS_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_scan_down, dim3(n_part, batch), dim3(kScanBlock), 0, s, in, n, mode, scratch, out, n_part,
-                     stride);
+                     stride, pstride);
   CS_CHECK_LAUNCH();
 }
 
 void launch_exclusive_scan(const int* in, int* out, long n, int mode, int* scratch, hipStream_t s) {
-  exclusive_scan_batch(in, out, n, mode, scratch, s, 1, 0);
+  exclusive_scan_batch(in, out, n, mode, scratch, s, 1, 0, 0);
+}
+
+void launch_exclusive_scan_pair(const int* in, int* out, long n, long stride, int* scratch, hipStream_t s) {
+  exclusive_scan_batch(in, out, n, 0, scratch, s, 2, stride, scan_scratch_ints(n));
 }
 
 // ---------------------------------------------------------------------------
@@ -216,7 +221,7 @@ void launch_radix_sort_batch(KV* data, KV* alt, long n, int bits, int* scratch, 
     CS_CHECK_LAUNCH();
     // the scan's input, output and partials sit at the same offsets in
     // every sort's scratch
-    exclusive_scan_batch(counts, offsets, table, 0, scan_tmp, s, batch, sstride);
+    exclusive_scan_batch(counts, offsets, table, 0, scan_tmp, s, batch, sstride, sstride);
     hipLaunchKernelGGL(k_radix_scatter, dim3(nb, batch), dim3(kRadixBlock), 0, s, src, dst, n, shift, offsets, nb,
                        data_stride, sstride);
     CS_CHECK_LAUNCH();

@@ -398,14 +398,15 @@ void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash
   int* list[3][2];
   for (int a = 0; a < 3; ++a)
     for (int k = 0; k < 2; ++k) list[a][k] = work + (size_t)(2 * a + k) * n;
-  // the scans write m + 1 outputs (the total last), so each holds n + 1
+  // the scans write m + 1 outputs (the total last), so each holds n + 1;
+  // fc and sc lie n + 1 after fb and sb, so one batched scan does both
   int* tag = work + 6 * (size_t)n;
   int* fb = work + 7 * (size_t)n;
-  int* fc = work + 8 * (size_t)n;
-  int* sb = work + 9 * (size_t)n;
-  int* sc = work + 10 * (size_t)n + 1;
-  int* at_b = work + 11 * (size_t)n + 2 - 1;   // indexed by heap node 1..n
-  int* at_c = work + 12 * (size_t)n + 2 - 1;
+  int* fc = fb + (size_t)n + 1;
+  int* sb = work + 9 * (size_t)n + 2;
+  int* sc = sb + (size_t)n + 1;
+  int* at_b = work + 11 * (size_t)n + 4 - 1;   // indexed by heap node 1..n
+  int* at_c = work + 12 * (size_t)n + 4 - 1;
   // the three presorted lists, sorted side by side (FP64 coordinates: low
   // key word, then high)
   const dim3 g3(ceil_div(n, kBlock), 3);
@@ -436,8 +437,7 @@ void launch_tree_build(const V4<T>* pos, int n, int n_dev, V4<T>* tree, KV* hash
       hipLaunchKernelGGL(k_tree_flags, gm, dim3(kBlock), 0, s, list[b][cur[b]], list[c][cur[c]], tag, fb, fc, at_b,
                          at_c, m, L);
       CS_CHECK_LAUNCH();
-      launch_exclusive_scan(fb, sb, m, 0, scan_scratch, s);
-      launch_exclusive_scan(fc, sc, m, 0, scan_scratch, s);
+      launch_exclusive_scan_pair(fb, sb, m, (long)n + 1, scan_scratch, s);
       hipLaunchKernelGGL(k_tree_regroup, gm, dim3(kBlock), 0, s, list[b][cur[b]], list[b][cur[b] ^ 1], sb, at_b,
                          list[c][cur[c]], list[c][cur[c] ^ 1], sc, at_c, tag, m, L, n);
       CS_CHECK_LAUNCH();
