@@ -414,7 +414,7 @@ def main():
                           "GBps": 2 * args.copy_calibration / ms / 1e6}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

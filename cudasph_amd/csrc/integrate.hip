@@ -601,8 +601,8 @@ void launch_copy16(const F4* src, F4* dst, long n, hipStream_t s) {
 }
 
 // FP32 FMA issue-rate probe: 8 independent chains per lane with runtime
-// coefficients (nothing folds), single-lane v_fma_f32 (this build emits no
-// packed FP32), 2 flops per FMA; each lane writes its sum
+// coefficients (nothing folds), single-lane v_fma_f32 (the library is
+// compiled without packed FP32), 2 flops per FMA; each lane writes its sum
 template <int FORM>
 __global__ __launch_bounds__(kBlock) void k_fma_peak(float* __restrict__ out, int iters, float b, float c) {
   constexpr int kChains = FORM ? 16 : 8;
